@@ -1,0 +1,1069 @@
+// HIP half of libnebula.so: device objects, the tracking kernels and the
+// device side of the C ABI (include/nebula.h).
+//
+// Particle buffer design.  Particles in flight live in two structure-of-arrays
+// buffers of `slots` entries used ping-pong.  In the buffer being read,
+// survivors of the previous launch occupy [0, n_surv) and secondary electrons
+// [slots - n_se, slots).  A launch of k_step moves every particle through up
+// to K steps (init + event each), then appends it to the other buffer:
+// survivors from the front, new secondaries from the back, each slot taken by
+// a wavefront ballot + prefix popcount + one atomic per wave.  Dead particles
+// are simply not copied, so the next launch reads a dense range.  A
+// one-thread control kernel adopts the output counts and plans the refill of
+// primaries from a device-resident queue; all sizes live in device memory, so
+// an iteration never waits for the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../host/host.hpp"
+#include "physics.hpp"
+
+using namespace nbl;
+using namespace nbl::dev;
+
+#define HIP_OK(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) \
+	return fail(NBL_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); } while (0)
+
+// ------------------------------------------------------------- buffers --
+struct Bufs {
+	float *px, *py, *pz, *dx, *dy, *dz, *E;
+	int *mat, *last_tri;
+	uint32_t *tag, *step, *flags;
+	uint64_t* sid;
+};
+
+struct Ctrl {
+	uint32_t buf_in;        // buffer the next k_step reads
+	uint32_t n_surv, n_se;  // particles in buf_in
+	uint32_t surv_ctr, se_ctr, se_limit;  // output counters of the running k_step
+	uint32_t stepped;       // a k_step ran since the last k_ctrl
+	uint32_t refill_n, refill_dst;
+	uint32_t target;        // particles in flight to aim for
+	uint32_t det_ctr;       // detected records written (may overshoot det_cap)
+	uint32_t pad0;
+	uint64_t refill_src;
+	uint64_t queue_n, queue_next;
+	uint64_t steps;         // particle-steps (init calls on active particles)
+};
+
+struct StepArgs {
+	World W;
+	Bufs buf[2];
+	Ctrl* ctrl;
+	nbl_detected* det;
+	uint32_t det_cap;
+	uint32_t slots;
+	uint32_t K;
+};
+
+__device__ __forceinline__ State load_state(const Bufs& b, uint32_t i)
+{
+	State s;
+	s.p.pos = { b.px[i], b.py[i], b.pz[i] };
+	s.p.dir = { b.dx[i], b.dy[i], b.dz[i] };
+	s.p.E = b.E[i];
+	s.mat = b.mat[i];
+	s.last_tri = b.last_tri[i];
+	s.tag = b.tag[i];
+	s.step = b.step[i];
+	s.flags = b.flags[i];
+	s.sid = b.sid[i];
+	return s;
+}
+
+__device__ __forceinline__ void store_state(const Bufs& b, uint32_t i, const State& s)
+{
+	b.px[i] = s.p.pos.x; b.py[i] = s.p.pos.y; b.pz[i] = s.p.pos.z;
+	b.dx[i] = s.p.dir.x; b.dy[i] = s.p.dir.y; b.dz[i] = s.p.dir.z;
+	b.E[i] = s.p.E;
+	b.mat[i] = s.mat;
+	b.last_tri[i] = s.last_tri;
+	b.tag[i] = s.tag;
+	b.step[i] = s.step;
+	b.flags[i] = s.flags;
+	b.sid[i] = s.sid;
+}
+
+// Slot allocation for the lanes of a wavefront with `want` set: one atomic
+// per wave, ranks by prefix popcount of the ballot.
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t* ctr, bool want)
+{
+	const uint64_t m = __ballot(want);
+	if (m == 0)
+		return 0;
+	const uint32_t lane = __lane_id();
+	const int leader = __ffsll((unsigned long long)m) - 1;
+	uint32_t base = 0;
+	if ((int)lane == leader)
+		base = atomicAdd(ctr, (uint32_t)__popcll(m));
+	base = (uint32_t)__shfl((int)base, leader);
+	return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// ------------------------------------------------------------- kernels --
+// Adopt the previous launch's output and plan the refill from the queue.
+__global__ void k_ctrl(Ctrl* c, uint32_t slots)
+{
+	if (threadIdx.x != 0 || blockIdx.x != 0)
+		return;
+	if (c->stepped) {
+		c->buf_in ^= 1u;
+		c->n_surv = c->surv_ctr;
+		c->n_se = min(c->se_ctr, c->se_limit);
+		c->stepped = 0;
+	}
+	const uint32_t active = c->n_surv + c->n_se;
+	uint32_t room = c->target > active ? c->target - active : 0u;
+	room = min(room, slots - active);
+	const uint64_t left = c->queue_n - c->queue_next;
+	const uint32_t m = (uint32_t)min<uint64_t>(left, (uint64_t)room);
+	c->refill_src = c->queue_next;
+	c->refill_dst = c->n_surv;
+	c->refill_n = m;
+	c->queue_next += m;
+	c->n_surv += m;
+	c->se_limit = slots - (c->n_surv + c->n_se);
+	c->surv_ctr = 0;
+	c->se_ctr = 0;
+	c->stepped = 1;
+}
+
+// Move planned primaries from the queue into the particle buffer
+// (gpu_driver.inl:367-389 / gpu_particle_manager.inl:342-353: vacuum
+// material, no last triangle).
+__global__ __launch_bounds__(256) void k_refill(Ctrl* c, Bufs b0, Bufs b1,
+	const nbl_particle28* __restrict__ q, const uint32_t* __restrict__ qtag,
+	const uint64_t* __restrict__ qsid)
+{
+	const uint32_t n = c->refill_n;
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const Bufs& b = c->buf_in ? b1 : b0;
+	const uint64_t src = c->refill_src + i;
+	const uint32_t dst = c->refill_dst + i;
+	const nbl_particle28 p = q[src];
+	State s;
+	s.p.pos = { p.pos[0], p.pos[1], p.pos[2] };
+	s.p.dir = { p.dir[0], p.dir[1], p.dir[2] };
+	s.p.E = p.kin_energy;
+	s.mat = MAT_VACUUM;
+	s.last_tri = -1;
+	s.tag = qtag[src];
+	s.step = 0;
+	s.flags = 0;
+	s.sid = qsid[src];
+	store_state(b, dst, s);
+}
+
+__global__ __launch_bounds__(256) void k_step(StepArgs a)
+{
+	Ctrl* C = a.ctrl;
+	const uint32_t bi = C->buf_in;
+	const Bufs in = bi ? a.buf[1] : a.buf[0];
+	const Bufs out = bi ? a.buf[0] : a.buf[1];
+	const uint32_t n_surv = C->n_surv, n_in = n_surv + C->n_se;
+	const uint32_t B = a.slots, se_limit = C->se_limit, det_cap = a.det_cap;
+	const World& W = a.W;
+	uint32_t my_steps = 0;
+	const uint32_t stride = gridDim.x * blockDim.x;
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_in; i += stride) {
+		const uint32_t slot = (i < n_surv) ? i : (B - 1u - (i - n_surv));
+		State s = load_state(in, slot);
+		bool keep = true;
+		for (uint32_t k = 0; k < a.K; ++k) {
+			int oc = ALIVE, ev = EV_NONE, hit = -1;
+			Rng rng;
+			rng_start(rng, W.key0, W.key1, s.sid, s.step);
+			State pre = s;
+			Particle se;
+			bool se_made = false;
+			if (s.flags & FL_PEND_DET) {
+				oc = DETECTED;
+			} else {
+				if (s.flags & FL_PEND_INEL) {
+					// resume an inelastic event whose path draws (0, 1) were
+					// taken when it was sampled
+					s.flags &= ~FL_PEND_INEL;
+					ev = EV_INELASTIC;
+					rng_seek(rng, 2);
+				} else {
+					oc = step_init(W, s, rng, ev, hit);
+					++my_steps;
+				}
+				pre = s;
+				if (oc == ALIVE)
+					oc = step_event(W, s, rng, ev, hit, se, se_made);
+			}
+			bool stop = false;
+			const uint32_t sslot = wave_alloc(&C->se_ctr, se_made);
+			if (se_made) {
+				if (sslot < se_limit) {
+					State ns;
+					ns.p = se;
+					ns.mat = s.mat;
+					ns.last_tri = -1;
+					ns.tag = s.tag;
+					ns.step = 0;
+					ns.flags = 0;
+					ns.sid = se_sid(s.sid, s.step);
+					store_state(out, B - 1u - sslot, ns);
+				} else {
+					// no room: keep the state before the event and retry it
+					s = pre;
+					s.flags |= FL_PEND_INEL;
+					stop = true;
+				}
+			}
+			const bool det = (oc == DETECTED);
+			const uint32_t dslot = wave_alloc(&C->det_ctr, det);
+			if (det) {
+				if (dslot < det_cap) {
+					nbl_detected r;
+					r.pos[0] = s.p.pos.x; r.pos[1] = s.p.pos.y; r.pos[2] = s.p.pos.z;
+					r.dir[0] = s.p.dir.x; r.dir[1] = s.p.dir.y; r.dir[2] = s.p.dir.z;
+					r.kin_energy = s.p.E;
+					r.tag = s.tag;
+					r.sid = s.sid;
+					a.det[dslot] = r;
+					keep = false;
+				} else {
+					s.flags |= FL_PEND_DET;
+					stop = true;
+				}
+			}
+			if (oc == TERMINATED)
+				keep = false;
+			if (!keep || stop)
+				break;
+			s.step++;
+		}
+		const uint32_t oslot = wave_alloc(&C->surv_ctr, keep);
+		if (keep)
+			store_state(out, oslot, s);
+	}
+	// particle-step count: wave reduction, one atomic per wave
+	uint32_t v = my_steps;
+	for (int off = 32; off > 0; off >>= 1)
+		v += (uint32_t)__shfl_xor((int)v, off);
+	if (__lane_id() == 0 && v)
+		atomicAdd((unsigned long long*)&C->steps, (unsigned long long)v);
+}
+
+__global__ void k_reset_det(Ctrl* c)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0)
+		c->det_ctr = 0;
+}
+
+// Test hook: batched propagate.
+__global__ __launch_bounds__(256) void k_query(Geometry g, const float* pos, const float* dir,
+	const float* dist, const int* ignore_tri, const int* ignore_mat, int* out_tri,
+	float* out_dist, uint32_t n)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const F3 p = { pos[3 * i], pos[3 * i + 1], pos[3 * i + 2] };
+	const F3 d = { dir[3 * i], dir[3 * i + 1], dir[3 * i + 2] };
+	float od;
+	out_tri[i] = propagate(g, p, d, dist[i], ignore_tri[i], ignore_mat[i], od);
+	out_dist[i] = od;
+}
+
+// Test hook: table lookups.
+__global__ __launch_bounds__(256) void k_probe(Material m, int which, int mode, const float* x,
+	const float* y, const float* z, float* out, uint32_t n)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	float r = 0.0f;
+	switch (which) {
+	case 0: r = t1_get(m.el_imfp, x[i]); break;
+	case 2: r = t1_get(m.in_imfp, x[i]); break;
+	case 1: case 3: case 6: {
+		const Table2& t = which == 1 ? m.el_icdf : which == 3 ? m.in_icdf : m.ki_ion;
+		r = mode == 0 ? t2_get(t, x[i], y[i]) : mode == 1 ? t2_get_rounddown(t, x[i], y[i])
+			: t2_get_nearest(t, x[i], y[i]);
+		break;
+	}
+	case 4: case 5: {
+		const Table3& t = which == 4 ? m.pn_q : m.pn_ion;
+		r = mode == 0 ? t3_get(t, x[i], y[i], z[i]) : t3_get_rounddown(t, x[i], y[i], z[i]);
+		break;
+	}
+	}
+	out[i] = r;
+}
+
+// Test hook: one step of independent states.
+__global__ __launch_bounds__(256) void k_step_states(World W, nbl_step_state* st, int* event,
+	int* outcome, nbl_step_state* se_out, int* se_made_out, uint32_t n)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	nbl_step_state h = st[i];
+	State s;
+	s.p.pos = { h.pos[0], h.pos[1], h.pos[2] };
+	s.p.dir = { h.dir[0], h.dir[1], h.dir[2] };
+	s.p.E = h.kin_energy;
+	s.mat = h.mat; s.last_tri = h.last_tri; s.tag = h.tag; s.step = h.step; s.flags = h.flags;
+	s.sid = h.sid;
+	Rng rng;
+	rng_start(rng, W.key0, W.key1, s.sid, s.step);
+	int ev = EV_NONE, hit = -1;
+	int oc = step_init(W, s, rng, ev, hit);
+	Particle se;
+	bool made = false;
+	if (oc == ALIVE)
+		oc = step_event(W, s, rng, ev, hit, se, made);
+	event[i] = ev;
+	outcome[i] = oc;
+	se_made_out[i] = made ? 1 : 0;
+	if (made) {
+		nbl_step_state o;
+		o.pos[0] = se.pos.x; o.pos[1] = se.pos.y; o.pos[2] = se.pos.z;
+		o.dir[0] = se.dir.x; o.dir[1] = se.dir.y; o.dir[2] = se.dir.z;
+		o.kin_energy = se.E;
+		o.mat = s.mat; o.last_tri = -1; o.tag = s.tag; o.step = 0; o.flags = 0;
+		o.sid = se_sid(s.sid, s.step);
+		se_out[i] = o;
+	}
+	h.pos[0] = s.p.pos.x; h.pos[1] = s.p.pos.y; h.pos[2] = s.p.pos.z;
+	h.dir[0] = s.p.dir.x; h.dir[1] = s.p.dir.y; h.dir[2] = s.p.dir.z;
+	h.kin_energy = s.p.E;
+	h.mat = s.mat; h.last_tri = s.last_tri; h.step = s.step + 1;
+	st[i] = h;
+}
+
+// ------------------------------------------------------- host objects --
+struct nbl_device {
+	int ordinal;
+	hipStream_t stream;
+};
+
+struct nbl_geometry {
+	nbl_device* dev;
+	int* od;
+	float4* tris;
+	Geometry g;
+	int max_material;   // largest physical material index named by a triangle
+};
+
+struct nbl_material {
+	nbl_device* dev;
+	float* tables[7];
+	Material m;
+};
+
+struct nbl_driver {
+	nbl_device* dev;
+	hipStream_t stream;
+	World W;
+	Material* d_mats;
+	uint32_t capacity, slots, K, det_cap;
+	Bufs buf[2];
+	void* buf_mem[2];
+	Ctrl* ctrl;
+	Ctrl* h_ctrl;
+	nbl_detected* det;
+	nbl_particle28* q_part;
+	uint32_t* q_tag;
+	uint64_t* q_sid;
+	uint64_t q_cap, q_n;
+	uint32_t next_tag;
+	bool timing;
+	std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
+	double step_ms;
+	uint64_t step_launches;
+};
+
+namespace {
+
+int set_device(const nbl_device* dev)
+{
+	HIP_OK(hipSetDevice(dev->ordinal));
+	return NBL_OK;
+}
+
+template <typename T>
+int dev_alloc(T** p, size_t count)
+{
+	*p = nullptr;
+	if (count == 0)
+		count = 1;
+	hipError_t e = hipMalloc((void**)p, sizeof(T) * count);
+	if (e != hipSuccess) {
+		*p = nullptr;
+		return fail(NBL_ERR_ALLOC, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+	}
+	return NBL_OK;
+}
+
+Table1 tab1(const nbl_table& t, const float* d) { return { d, t.dims[0], t.min[0], t.step[0] }; }
+Table2 tab2(const nbl_table& t, const float* d)
+{
+	return { d, t.dims[0], t.dims[1], t.min[0], t.step[0], t.min[1], t.step[1] };
+}
+Table3 tab3(const nbl_table& t, const float* d)
+{
+	return { d, t.dims[0], t.dims[1], t.dims[2], t.min[0], t.step[0], t.min[1], t.step[1], t.min[2], t.step[2] };
+}
+
+size_t tcount(const nbl_table& t)
+{
+	if (!t.data || t.ndim <= 0)
+		return 0;
+	size_t c = 1;
+	for (int a = 0; a < t.ndim; ++a)
+		c *= (size_t)t.dims[a];
+	return c;
+}
+
+uint32_t blocks_for(uint64_t n, uint32_t bs = 256)
+{
+	const uint64_t b = (n + bs - 1) / bs;
+	return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, 1u << 20));
+}
+
+int alloc_bufs(Bufs& b, void*& mem, uint32_t slots)
+{
+	const size_t n = slots;
+	const size_t bytes = n * (7 * 4 + 2 * 4 + 3 * 4 + 8) + 256;
+	char* p = nullptr;
+	if (hipMalloc((void**)&p, bytes) != hipSuccess)
+		return fail(NBL_ERR_ALLOC, "hipMalloc of the particle buffers failed");
+	mem = p;
+	// sid first (8-byte aligned), then the 4-byte fields
+	b.sid = (uint64_t*)p; p += 8 * n;
+	float** f[7] = { &b.px, &b.py, &b.pz, &b.dx, &b.dy, &b.dz, &b.E };
+	for (int k = 0; k < 7; ++k) { *f[k] = (float*)p; p += 4 * n; }
+	b.mat = (int*)p; p += 4 * n;
+	b.last_tri = (int*)p; p += 4 * n;
+	b.tag = (uint32_t*)p; p += 4 * n;
+	b.step = (uint32_t*)p; p += 4 * n;
+	b.flags = (uint32_t*)p; p += 4 * n;
+	return NBL_OK;
+}
+
+int collect_timing(nbl_driver* d)
+{
+	for (auto& pr : d->ev_used) {
+		float ms = 0;
+		HIP_OK(hipEventElapsedTime(&ms, pr.first, pr.second));
+		d->step_ms += ms;
+		d->ev_free.push_back(pr);
+	}
+	d->ev_used.clear();
+	return NBL_OK;
+}
+
+int read_ctrl(nbl_driver* d)
+{
+	HIP_OK(hipMemcpyAsync(d->h_ctrl, d->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, d->stream));
+	HIP_OK(hipStreamSynchronize(d->stream));
+	if (d->timing)
+		return collect_timing(d);
+	return NBL_OK;
+}
+
+uint32_t active_count(const Ctrl& c)
+{
+	if (c.stepped)
+		return c.surv_ctr + std::min(c.se_ctr, c.se_limit);
+	return c.n_surv + c.n_se;
+}
+
+} // namespace
+
+// ================================================================ C ABI ==
+extern "C" {
+
+int nbl_device_open(int ordinal, nbl_device** out)
+{
+	clear_error();
+	if (!out)
+		return fail(NBL_ERR_ARG, "null argument");
+	int n = 0;
+	HIP_OK(hipGetDeviceCount(&n));
+	if (ordinal < 0 || ordinal >= n)
+		return fail(NBL_ERR_ARG, "no such GPU ordinal " + std::to_string(ordinal));
+	HIP_OK(hipSetDevice(ordinal));
+	auto* d = new nbl_device;
+	d->ordinal = ordinal;
+	hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+	if (e != hipSuccess) {
+		delete d;
+		return fail(NBL_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+	}
+	*out = d;
+	return NBL_OK;
+}
+
+int nbl_device_close(nbl_device* d)
+{
+	clear_error();
+	if (!d)
+		return NBL_OK;
+	hipSetDevice(d->ordinal);
+	hipStreamSynchronize(d->stream);
+	hipStreamDestroy(d->stream);
+	delete d;
+	return NBL_OK;
+}
+
+int nbl_device_synchronize(nbl_device* d)
+{
+	clear_error();
+	if (!d)
+		return fail(NBL_ERR_ARG, "null device");
+	HIP_OK(hipSetDevice(d->ordinal));
+	HIP_OK(hipStreamSynchronize(d->stream));
+	return NBL_OK;
+}
+
+int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const nbl_triangle44* tris,
+	int64_t ntri, const float center[3], const float halfsize[3], nbl_geometry** out)
+{
+	clear_error();
+	if (!dev || !od || n_od < 9 || !tris || ntri < 0 || !center || !halfsize || !out)
+		return fail(NBL_ERR_ARG, "invalid geometry arguments");
+	// Walk the whole table once: every child offset must point at a node
+	// inside it and every leaf entry at a triangle, so that the traversal
+	// kernel cannot read out of bounds.
+	{
+		std::vector<uint8_t> seen((size_t)n_od, 0);
+		std::vector<int64_t> stack{ 0 };
+		seen[0] = 1;
+		while (!stack.empty()) {
+			const int64_t node = stack.back();
+			stack.pop_back();
+			if (node + 8 > n_od)
+				return fail(NBL_ERR_FORMAT, "octree node out of range at " + std::to_string(node));
+			for (int o = 0; o < 8; ++o) {
+				const int64_t v = od[node + o];
+				if (v > 0) {
+					if (v >= n_od || seen[v])
+						return fail(NBL_ERR_FORMAT, "bad octree child offset at " + std::to_string(node + o));
+					seen[v] = 1;
+					stack.push_back(v);
+				} else {
+					int64_t p = -v;
+					if (p <= 0 || p >= n_od)
+						return fail(NBL_ERR_FORMAT, "bad octree leaf offset at " + std::to_string(node + o));
+					for (;; ++p) {
+						if (p >= n_od)
+							return fail(NBL_ERR_FORMAT, "unterminated octree leaf");
+						if (od[p] < 0)
+							break;
+						if (od[p] >= ntri)
+							return fail(NBL_ERR_FORMAT, "octree leaf names triangle " + std::to_string(od[p]));
+					}
+				}
+			}
+		}
+	}
+	int rc = set_device(dev);
+	if (rc) return rc;
+	auto* g = new nbl_geometry();
+	g->dev = dev;
+	if ((rc = dev_alloc(&g->od, (size_t)n_od)) || (rc = dev_alloc(&g->tris, (size_t)ntri * 3))) {
+		hipFree(g->od);
+		delete g;
+		return rc;
+	}
+	std::vector<float4> packed((size_t)ntri * 3);
+	for (int64_t i = 0; i < ntri; ++i) {
+		const nbl_triangle44& t = tris[i];
+		float ib, ob;
+		std::memcpy(&ib, &t.material_in, 4);
+		std::memcpy(&ob, &t.material_out, 4);
+		packed[3 * i] = make_float4(t.r0[0], t.r0[1], t.r0[2], t.e1[0]);
+		packed[3 * i + 1] = make_float4(t.e1[1], t.e1[2], t.e2[0], t.e2[1]);
+		packed[3 * i + 2] = make_float4(t.e2[2], ib, ob, 0.0f);
+	}
+	HIP_OK(hipMemcpy(g->od, od, sizeof(int32_t) * n_od, hipMemcpyHostToDevice));
+	HIP_OK(hipMemcpy(g->tris, packed.data(), sizeof(float4) * packed.size(), hipMemcpyHostToDevice));
+	g->g.od = g->od;
+	g->g.tris = g->tris;
+	g->g.center = { center[0], center[1], center[2] };
+	g->g.halfsize = { halfsize[0], halfsize[1], halfsize[2] };
+	// octree.inl:298-300: 2 |halfsize|, CPU expression order
+	const float h2 = halfsize[0] * halfsize[0] + halfsize[1] * halfsize[1] + halfsize[2] * halfsize[2];
+	g->g.max_extent = 2 * std::sqrt(h2);
+	g->g.ntri = (int)ntri;
+	g->max_material = -1;
+	for (int64_t i = 0; i < ntri; ++i)
+		g->max_material = std::max(g->max_material, std::max(tris[i].material_in, tris[i].material_out));
+	*out = g;
+	return NBL_OK;
+}
+
+int nbl_geometry_destroy(nbl_geometry* g)
+{
+	clear_error();
+	if (!g)
+		return NBL_OK;
+	hipSetDevice(g->dev->ordinal);
+	hipFree(g->od);
+	hipFree(g->tris);
+	delete g;
+	return NBL_OK;
+}
+
+int nbl_material_upload(nbl_device* dev, const nbl_material_host* m, nbl_material** out)
+{
+	clear_error();
+	if (!dev || !m || !out)
+		return fail(NBL_ERR_ARG, "null argument");
+	int rc = set_device(dev);
+	if (rc) return rc;
+	auto* mm = new nbl_material();
+	mm->dev = dev;
+	const nbl_table* ts[7] = { &m->el_imfp, &m->el_icdf, &m->in_imfp, &m->in_icdf, &m->pn_q, &m->pn_ion, &m->ki_ion };
+	for (int k = 0; k < 7; ++k) {
+		const size_t c = tcount(*ts[k]);
+		mm->tables[k] = nullptr;
+		if (!c)
+			continue;
+		if ((rc = dev_alloc(&mm->tables[k], c))) {
+			for (int j = 0; j < k; ++j) hipFree(mm->tables[j]);
+			delete mm;
+			return rc;
+		}
+		HIP_OK(hipMemcpy(mm->tables[k], ts[k]->data, sizeof(float) * c, hipMemcpyHostToDevice));
+	}
+	Material& d = mm->m;
+	d.barrier = m->barrier;
+	d.phonon_loss = m->phonon_loss;
+	d.recoil_const = m->recoil_const;
+	d.fermi = m->fermi;
+	d.band_gap = m->band_gap;
+	d.inel_model = m->inel_model;
+	d.el_imfp = tab1(m->el_imfp, mm->tables[0]);
+	d.el_icdf = tab2(m->el_icdf, mm->tables[1]);
+	d.in_imfp = tab1(m->in_imfp, mm->tables[2]);
+	d.in_icdf = tab2(m->in_icdf, mm->tables[3]);
+	d.pn_q = tab3(m->pn_q, mm->tables[4]);
+	d.pn_ion = tab3(m->pn_ion, mm->tables[5]);
+	d.ki_ion = tab2(m->ki_ion, mm->tables[6]);
+	if (m->inel_model == NBL_INEL_PENN && (!mm->tables[4] || !mm->tables[5]))
+		return fail(NBL_ERR_FORMAT, "full Penn material without q / binding tables");
+	if (m->inel_model == NBL_INEL_KIEFT && !mm->tables[6])
+		return fail(NBL_ERR_FORMAT, "Kieft material without binding table");
+	*out = mm;
+	return NBL_OK;
+}
+
+int nbl_material_destroy(nbl_material* m)
+{
+	clear_error();
+	if (!m)
+		return NBL_OK;
+	hipSetDevice(m->dev->ordinal);
+	for (float* t : m->tables)
+		hipFree(t);
+	delete m;
+	return NBL_OK;
+}
+
+int nbl_driver_destroy(nbl_driver* d)
+{
+	clear_error();
+	if (!d)
+		return NBL_OK;
+	hipSetDevice(d->dev->ordinal);
+	hipStreamSynchronize(d->stream);
+	for (auto& pr : d->ev_used) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
+	for (auto& pr : d->ev_free) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
+	hipFree(d->buf_mem[0]);
+	hipFree(d->buf_mem[1]);
+	hipFree(d->ctrl);
+	hipHostFree(d->h_ctrl);
+	hipFree(d->det);
+	hipFree(d->d_mats);
+	hipFree(d->q_part);
+	hipFree(d->q_tag);
+	hipFree(d->q_sid);
+	hipStreamDestroy(d->stream);
+	delete d;
+	return NBL_OK;
+}
+
+int nbl_driver_create(nbl_device* dev, uint32_t capacity, const nbl_geometry* geom,
+	const nbl_material* const* mats, int32_t n_mats, uint32_t physics_flags,
+	float energy_threshold, uint64_t seed, uint32_t steps_per_launch, nbl_driver** out)
+{
+	clear_error();
+	if (!dev || !geom || !out || (n_mats > 0 && !mats) || n_mats < 0)
+		return fail(NBL_ERR_ARG, "invalid driver arguments");
+	if (capacity < 64 || capacity > (1u << 30))
+		return fail(NBL_ERR_CAPACITY, "capacity must be in [64, 2^30]");
+	if (steps_per_launch < 1 || steps_per_launch > 1024)
+		return fail(NBL_ERR_ARG, "steps_per_launch must be in [1, 1024]");
+	for (int i = 0; i < n_mats; ++i)
+		if (!mats[i] || mats[i]->dev != dev)
+			return fail(NBL_ERR_ARG, "material " + std::to_string(i) + " missing or on another device");
+	// gpu_main.cu:313-335: every material the geometry names must be given
+	if (geom->max_material >= n_mats)
+		return fail(NBL_ERR_ARG, "not enough materials provided for this geometry: expected " +
+			std::to_string(geom->max_material + 1) + ", got " + std::to_string(n_mats));
+	int rc = set_device(dev);
+	if (rc) return rc;
+	auto* d = new nbl_driver();
+	d->dev = dev;
+	d->capacity = capacity;
+	d->slots = 2 * capacity;
+	d->K = steps_per_launch;
+	d->det_cap = capacity;
+	auto cleanup = [&](int code) { nbl_driver_destroy(d); return code; };
+	hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+	if (e != hipSuccess) {
+		delete d;
+		return fail(NBL_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+	}
+	if ((rc = alloc_bufs(d->buf[0], d->buf_mem[0], d->slots))) return cleanup(rc);
+	if ((rc = alloc_bufs(d->buf[1], d->buf_mem[1], d->slots))) return cleanup(rc);
+	if ((rc = dev_alloc(&d->ctrl, 1))) return cleanup(rc);
+	if (hipHostMalloc((void**)&d->h_ctrl, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess)
+		return cleanup(fail(NBL_ERR_ALLOC, "hipHostMalloc failed"));
+	if ((rc = dev_alloc(&d->det, d->det_cap))) return cleanup(rc);
+	if ((rc = dev_alloc(&d->d_mats, (size_t)std::max(1, n_mats)))) return cleanup(rc);
+	std::vector<Material> hm((size_t)std::max(1, n_mats));
+	for (int i = 0; i < n_mats; ++i)
+		hm[i] = mats[i]->m;
+	if (hipMemcpy(d->d_mats, hm.data(), sizeof(Material) * hm.size(), hipMemcpyHostToDevice) != hipSuccess)
+		return cleanup(fail(NBL_ERR_DEVICE, "material descriptor upload failed"));
+	Ctrl c;
+	std::memset(&c, 0, sizeof(c));
+	c.target = capacity;
+	if (hipMemcpy(d->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess)
+		return cleanup(fail(NBL_ERR_DEVICE, "control block upload failed"));
+	d->W.g = geom->g;
+	d->W.mats = d->d_mats;
+	d->W.nmats = n_mats;
+	d->W.flags = physics_flags;
+	d->W.threshold = energy_threshold;
+	d->W.key0 = (uint32_t)seed;
+	d->W.key1 = (uint32_t)(seed >> 32);
+	*out = d;
+	return NBL_OK;
+}
+
+int nbl_driver_push(nbl_driver* d, const nbl_particle28* particles, const uint32_t* tags,
+	uint32_t n, uint64_t sid_base, uint32_t* pushed)
+{
+	clear_error();
+	if (!d || (n && !particles))
+		return fail(NBL_ERR_ARG, "invalid push arguments");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	HIP_OK(hipStreamSynchronize(d->stream));
+	// device queue: grow by doubling, keeping the unread part
+	if (d->q_n + n > d->q_cap) {
+		uint64_t nc = std::max<uint64_t>(d->q_cap ? 2 * d->q_cap : 1024, d->q_n + n);
+		nbl_particle28* np; uint32_t* nt; uint64_t* ns;
+		if ((rc = dev_alloc(&np, nc)) || (rc = dev_alloc(&nt, nc)) || (rc = dev_alloc(&ns, nc)))
+			return rc;
+		if (d->q_n) {
+			HIP_OK(hipMemcpy(np, d->q_part, sizeof(nbl_particle28) * d->q_n, hipMemcpyDeviceToDevice));
+			HIP_OK(hipMemcpy(nt, d->q_tag, sizeof(uint32_t) * d->q_n, hipMemcpyDeviceToDevice));
+			HIP_OK(hipMemcpy(ns, d->q_sid, sizeof(uint64_t) * d->q_n, hipMemcpyDeviceToDevice));
+		}
+		hipFree(d->q_part); hipFree(d->q_tag); hipFree(d->q_sid);
+		d->q_part = np; d->q_tag = nt; d->q_sid = ns;
+		d->q_cap = nc;
+	}
+	std::vector<uint32_t> t(n);
+	std::vector<uint64_t> s(n);
+	for (uint32_t i = 0; i < n; ++i) {
+		t[i] = tags ? tags[i] : d->next_tag + i;
+		s[i] = sid_base + i;
+	}
+	d->next_tag += n;
+	HIP_OK(hipMemcpy(d->q_part + d->q_n, particles, sizeof(nbl_particle28) * n, hipMemcpyHostToDevice));
+	HIP_OK(hipMemcpy(d->q_tag + d->q_n, t.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+	HIP_OK(hipMemcpy(d->q_sid + d->q_n, s.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+	d->q_n += n;
+	const uint64_t qn = d->q_n;
+	HIP_OK(hipMemcpy(&d->ctrl->queue_n, &qn, sizeof(uint64_t), hipMemcpyHostToDevice));
+	if (pushed)
+		*pushed = n;
+	return NBL_OK;
+}
+
+int nbl_driver_iterate(nbl_driver* d, uint32_t n_iter)
+{
+	clear_error();
+	if (!d)
+		return fail(NBL_ERR_ARG, "null driver");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	StepArgs a;
+	a.W = d->W;
+	a.buf[0] = d->buf[0];
+	a.buf[1] = d->buf[1];
+	a.ctrl = d->ctrl;
+	a.det = d->det;
+	a.det_cap = d->det_cap;
+	a.slots = d->slots;
+	a.K = d->K;
+	const uint32_t g_refill = blocks_for(d->capacity);
+	const uint32_t g_step = blocks_for(d->capacity);
+	for (uint32_t it = 0; it < n_iter; ++it) {
+		hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(64), 0, d->stream, d->ctrl, d->slots);
+		hipLaunchKernelGGL(k_refill, dim3(g_refill), dim3(256), 0, d->stream, d->ctrl, d->buf[0],
+			d->buf[1], d->q_part, d->q_tag, d->q_sid);
+		std::pair<hipEvent_t, hipEvent_t> ev{};
+		if (d->timing) {
+			if (d->ev_free.empty()) {
+				HIP_OK(hipEventCreate(&ev.first));
+				HIP_OK(hipEventCreate(&ev.second));
+			} else {
+				ev = d->ev_free.back();
+				d->ev_free.pop_back();
+			}
+			HIP_OK(hipEventRecord(ev.first, d->stream));
+		}
+		hipLaunchKernelGGL(k_step, dim3(g_step), dim3(256), 0, d->stream, a);
+		if (d->timing) {
+			HIP_OK(hipEventRecord(ev.second, d->stream));
+			d->ev_used.push_back(ev);
+			d->step_launches++;
+		}
+	}
+	HIP_OK(hipGetLastError());
+	return NBL_OK;
+}
+
+int nbl_driver_counts(nbl_driver* d, uint32_t* running, uint32_t* detected, uint64_t* queued)
+{
+	clear_error();
+	if (!d)
+		return fail(NBL_ERR_ARG, "null driver");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	if ((rc = read_ctrl(d)))
+		return rc;
+	const Ctrl& c = *d->h_ctrl;
+	if (running) *running = active_count(c);
+	if (detected) *detected = std::min(c.det_ctr, d->det_cap);
+	if (queued) *queued = c.queue_n - c.queue_next;
+	return NBL_OK;
+}
+
+int nbl_driver_drain_detected(nbl_driver* d, nbl_detected* out, uint32_t cap, uint32_t* n)
+{
+	clear_error();
+	if (!d || !n || (cap && !out))
+		return fail(NBL_ERR_ARG, "invalid drain arguments");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	if ((rc = read_ctrl(d)))
+		return rc;
+	const uint32_t have = std::min(d->h_ctrl->det_ctr, d->det_cap);
+	if (have > cap) {
+		*n = have;
+		return fail(NBL_ERR_CAPACITY, "output array too small for " + std::to_string(have) + " records");
+	}
+	if (have)
+		HIP_OK(hipMemcpyAsync(out, d->det, sizeof(nbl_detected) * have, hipMemcpyDeviceToHost, d->stream));
+	hipLaunchKernelGGL(k_reset_det, dim3(1), dim3(64), 0, d->stream, d->ctrl);
+	HIP_OK(hipStreamSynchronize(d->stream));
+	*n = have;
+	return NBL_OK;
+}
+
+int nbl_driver_steps(nbl_driver* d, uint64_t* steps)
+{
+	clear_error();
+	if (!d || !steps)
+		return fail(NBL_ERR_ARG, "null argument");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	if ((rc = read_ctrl(d)))
+		return rc;
+	*steps = d->h_ctrl->steps;
+	return NBL_OK;
+}
+
+int nbl_driver_run(nbl_driver* d, nbl_detected* out, uint64_t cap, uint64_t* n_detected)
+{
+	clear_error();
+	if (!d || !n_detected || (cap && !out))
+		return fail(NBL_ERR_ARG, "invalid run arguments");
+	uint64_t total = 0;
+	uint32_t frame = 4;
+	std::vector<nbl_detected> tmp;
+	for (;;) {
+		int rc = nbl_driver_iterate(d, frame);
+		if (rc) return rc;
+		uint32_t running = 0, det = 0;
+		uint64_t queued = 0;
+		if ((rc = nbl_driver_counts(d, &running, &det, &queued)))
+			return rc;
+		if (det) {
+			uint32_t got = 0;
+			const uint64_t room = cap > total ? cap - total : 0;
+			if (room >= det) {
+				if ((rc = nbl_driver_drain_detected(d, out + total, det, &got)))
+					return rc;
+			} else {
+				tmp.resize(det);
+				if ((rc = nbl_driver_drain_detected(d, tmp.data(), det, &got)))
+					return rc;
+				if (room)
+					std::memcpy(out + total, tmp.data(), sizeof(nbl_detected) * room);
+			}
+			total += got;
+		}
+		if (running == 0 && queued == 0 && d->h_ctrl->det_ctr == 0)
+			break;
+		frame = std::min<uint32_t>(frame * 2, 32);
+	}
+	*n_detected = total;
+	if (total > cap)
+		return fail(NBL_ERR_CAPACITY, "more detected electrons than the output array holds");
+	return NBL_OK;
+}
+
+void* nbl_driver_stream(nbl_driver* d) { return d ? (void*)d->stream : nullptr; }
+
+int nbl_geometry_query(const nbl_geometry* g, const float* pos, const float* dir, const float* dist,
+	const int32_t* ignore_tri, const int32_t* ignore_mat, int32_t* out_tri, float* out_dist, uint32_t n)
+{
+	clear_error();
+	if (!g || (n && (!pos || !dir || !dist || !ignore_tri || !ignore_mat || !out_tri || !out_dist)))
+		return fail(NBL_ERR_ARG, "invalid query arguments");
+	if (!n)
+		return NBL_OK;
+	int rc = set_device(g->dev);
+	if (rc) return rc;
+	for (uint32_t i = 0; i < n; ++i)
+		if (ignore_tri[i] < -1 || ignore_tri[i] >= g->g.ntri)
+			return fail(NBL_ERR_ARG, "ignore_tri out of range");
+	float *dp, *dd, *ddist, *dout;
+	int *dit, *dim, *dtri;
+	if ((rc = dev_alloc(&dp, 3 * (size_t)n)) || (rc = dev_alloc(&dd, 3 * (size_t)n)) ||
+		(rc = dev_alloc(&ddist, n)) || (rc = dev_alloc(&dit, n)) || (rc = dev_alloc(&dim, n)) ||
+		(rc = dev_alloc(&dtri, n)) || (rc = dev_alloc(&dout, n)))
+		return rc;
+	hipStream_t s = g->dev->stream;
+	HIP_OK(hipMemcpyAsync(dp, pos, 12 * (size_t)n, hipMemcpyHostToDevice, s));
+	HIP_OK(hipMemcpyAsync(dd, dir, 12 * (size_t)n, hipMemcpyHostToDevice, s));
+	HIP_OK(hipMemcpyAsync(ddist, dist, 4 * (size_t)n, hipMemcpyHostToDevice, s));
+	HIP_OK(hipMemcpyAsync(dit, ignore_tri, 4 * (size_t)n, hipMemcpyHostToDevice, s));
+	HIP_OK(hipMemcpyAsync(dim, ignore_mat, 4 * (size_t)n, hipMemcpyHostToDevice, s));
+	hipLaunchKernelGGL(k_query, dim3(blocks_for(n)), dim3(256), 0, s, g->g, dp, dd, ddist, dit, dim,
+		dtri, dout, n);
+	HIP_OK(hipGetLastError());
+	HIP_OK(hipMemcpyAsync(out_tri, dtri, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+	HIP_OK(hipMemcpyAsync(out_dist, dout, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+	HIP_OK(hipStreamSynchronize(s));
+	hipFree(dp); hipFree(dd); hipFree(ddist); hipFree(dit); hipFree(dim); hipFree(dtri); hipFree(dout);
+	return NBL_OK;
+}
+
+int nbl_table_probe(const nbl_material* m, int32_t which, int32_t mode, const float* x,
+	const float* y, const float* z, float* out, uint32_t n)
+{
+	clear_error();
+	if (!m || which < 0 || which > 6 || mode < 0 || mode > 2 || (n && (!x || !out)))
+		return fail(NBL_ERR_ARG, "invalid probe arguments");
+	if (!m->tables[which])
+		return fail(NBL_ERR_ARG, "material has no such table");
+	if (!n)
+		return NBL_OK;
+	const bool two = (which == 1 || which == 3 || which == 6);
+	const bool three = (which == 4 || which == 5);
+	if ((two || three) && !y)
+		return fail(NBL_ERR_ARG, "y needed");
+	if (three && !z)
+		return fail(NBL_ERR_ARG, "z needed");
+	if (mode == 2 && !two)
+		return fail(NBL_ERR_ARG, "get_nearest exists for 2-D tables only");
+	int rc = set_device(m->dev);
+	if (rc) return rc;
+	float *dx, *dy, *dz, *dout;
+	if ((rc = dev_alloc(&dx, n)) || (rc = dev_alloc(&dy, n)) || (rc = dev_alloc(&dz, n)) ||
+		(rc = dev_alloc(&dout, n)))
+		return rc;
+	hipStream_t s = m->dev->stream;
+	HIP_OK(hipMemcpyAsync(dx, x, 4 * (size_t)n, hipMemcpyHostToDevice, s));
+	HIP_OK(hipMemcpyAsync(dy, y ? y : x, 4 * (size_t)n, hipMemcpyHostToDevice, s));
+	HIP_OK(hipMemcpyAsync(dz, z ? z : x, 4 * (size_t)n, hipMemcpyHostToDevice, s));
+	hipLaunchKernelGGL(k_probe, dim3(blocks_for(n)), dim3(256), 0, s, m->m, which, mode, dx, dy, dz, dout, n);
+	HIP_OK(hipGetLastError());
+	HIP_OK(hipMemcpyAsync(out, dout, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+	HIP_OK(hipStreamSynchronize(s));
+	hipFree(dx); hipFree(dy); hipFree(dz); hipFree(dout);
+	return NBL_OK;
+}
+
+int nbl_driver_step_states(nbl_driver* d, nbl_step_state* states, int32_t* event, int32_t* outcome,
+	nbl_step_state* se, int32_t* se_made, uint32_t n)
+{
+	clear_error();
+	if (!d || (n && (!states || !event || !outcome || !se || !se_made)))
+		return fail(NBL_ERR_ARG, "invalid step arguments");
+	if (!n)
+		return NBL_OK;
+	for (uint32_t i = 0; i < n; ++i) {
+		if (states[i].mat >= d->W.nmats || (states[i].mat < 0 && states[i].mat < -128))
+			return fail(NBL_ERR_ARG, "state material out of range");
+		if (states[i].last_tri < -1 || states[i].last_tri >= d->W.g.ntri)
+			return fail(NBL_ERR_ARG, "state last_tri out of range");
+	}
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	nbl_step_state *dst, *dse;
+	int *dev_ev, *dev_oc, *dev_made;
+	if ((rc = dev_alloc(&dst, n)) || (rc = dev_alloc(&dse, n)) || (rc = dev_alloc(&dev_ev, n)) ||
+		(rc = dev_alloc(&dev_oc, n)) || (rc = dev_alloc(&dev_made, n)))
+		return rc;
+	hipStream_t s = d->stream;
+	HIP_OK(hipMemcpyAsync(dst, states, sizeof(nbl_step_state) * n, hipMemcpyHostToDevice, s));
+	hipLaunchKernelGGL(k_step_states, dim3(blocks_for(n)), dim3(256), 0, s, d->W, dst, dev_ev, dev_oc,
+		dse, dev_made, n);
+	HIP_OK(hipGetLastError());
+	HIP_OK(hipMemcpyAsync(states, dst, sizeof(nbl_step_state) * n, hipMemcpyDeviceToHost, s));
+	HIP_OK(hipMemcpyAsync(se, dse, sizeof(nbl_step_state) * n, hipMemcpyDeviceToHost, s));
+	HIP_OK(hipMemcpyAsync(event, dev_ev, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+	HIP_OK(hipMemcpyAsync(outcome, dev_oc, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+	HIP_OK(hipMemcpyAsync(se_made, dev_made, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
+	HIP_OK(hipStreamSynchronize(s));
+	hipFree(dst); hipFree(dse); hipFree(dev_ev); hipFree(dev_oc); hipFree(dev_made);
+	return NBL_OK;
+}
+
+// Per-launch timing of k_step with HIP events on the driver's stream.
+int nbl_driver_set_timing(nbl_driver* d, int enable)
+{
+	clear_error();
+	if (!d)
+		return fail(NBL_ERR_ARG, "null driver");
+	d->timing = enable != 0;
+	d->step_ms = 0;
+	d->step_launches = 0;
+	return NBL_OK;
+}
+
+int nbl_driver_timing(nbl_driver* d, double* step_ms_total, uint64_t* step_launches)
+{
+	clear_error();
+	if (!d || !step_ms_total || !step_launches)
+		return fail(NBL_ERR_ARG, "null argument");
+	int rc = read_ctrl(d);
+	if (rc) return rc;
+	*step_ms_total = d->step_ms;
+	*step_launches = d->step_launches;
+	return NBL_OK;
+}
+
+} // extern "C"

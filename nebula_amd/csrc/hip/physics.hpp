@@ -1,0 +1,773 @@
+// Device-side tracking physics for gfx950.
+//
+// Every floating-point expression keeps the operation order of the reference
+// function cited beside it (paths relative to the reference's source/), and
+// the library is compiled with -ffp-contract=off, so each step is
+// bit-reproducible against a CPU evaluation of the same expressions.
+// Transcendentals are evaluated in double and rounded to float.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace nbl { namespace dev {
+
+constexpr float EPS = 10.0f * 1.1920928955078125e-07f;  // config/data_types.h:55
+constexpr float PI_F = 3.1415926535897932f;             // config/config.h:32
+// Bounds on the two unbounded loops of the reference (octree cell walk,
+// plasmon-decay rejection), so that every wave finishes.
+constexpr uint32_t MAX_CELLS = 1000000u;
+constexpr int MAX_REJECT = 1000;
+
+// Special material indices (core/material_manager.h:77-86).
+constexpr int MAT_NOP = -128, MAT_TERMINATOR = -127, MAT_DETECTOR = -126,
+	MAT_DETECTOR_LT50 = -125, MAT_DETECTOR_GE50 = -124, MAT_VACUUM = -123,
+	MAT_MIRROR = -122;
+
+// Physics option bits (same values as NBL_PHYS_* in nebula.h).
+constexpr uint32_t PH_EL_PHONON = 1u << 0, PH_EL_RECOIL = 1u << 1, PH_PN_SE = 1u << 2,
+	PH_KI_OPTICAL = 1u << 3, PH_KI_SE = 1u << 4, PH_KI_INST = 1u << 5, PH_KI_MOMCONS = 1u << 6,
+	PH_BI_QUANTUM = 1u << 7, PH_BI_REFRACT = 1u << 8, PH_BI_ABSORB = 1u << 9;
+
+enum { ALIVE = 0, TERMINATED = 1, DETECTED = 2 };
+enum { EV_NONE = 0, EV_INELASTIC = 1, EV_ELASTIC = 2, EV_INTERSECT = 3 };
+// Particle flags.
+constexpr uint32_t FL_PEND_INEL = 1u, FL_PEND_DET = 2u;
+
+struct F3 { float x, y, z; };
+
+// ------------------------------------------------------------------ math --
+__device__ __forceinline__ float m_log(float x) { return (float)log((double)x); }
+__device__ __forceinline__ float m_exp(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float m_expm1(float x) { return (float)expm1((double)x); }
+__device__ __forceinline__ float m_cbrt(float x) { return (float)cbrt((double)x); }
+__device__ __forceinline__ float m_sin(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float m_cos(float x) { return (float)cos((double)x); }
+__device__ __forceinline__ float m_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+
+// common/scalar_math.h:16-25; float->int conversions follow the x86 CPU
+// build for out-of-range and NaN inputs (INT_MIN).
+__device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+__device__ __forceinline__ float clampr(float i, float lo, float hi) { return i < lo ? lo : (i > hi ? hi : i); }
+__device__ __forceinline__ float fmin_std(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float fmax_std(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ int to_int_x86(float f)
+{
+	return (f >= -2147483648.0f && f < 2147483648.0f) ? (int)f : (int)0x80000000;
+}
+__device__ __forceinline__ int floorr(float x) { return to_int_x86(floorf(x)); }
+__device__ __forceinline__ int rintr(float x) { return to_int_x86(rintf(x)); }
+__device__ __forceinline__ float saturater(float x) { return clampr(x, 0.0f, 1.0f); }
+
+// common/vector_math.h
+__device__ __forceinline__ F3 v_add(F3 a, F3 b) { return { a.x + b.x, a.y + b.y, a.z + b.z }; }
+__device__ __forceinline__ F3 v_sub(F3 a, F3 b) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
+__device__ __forceinline__ F3 v_mul(F3 a, float b) { return { a.x * b, a.y * b, a.z * b }; }
+__device__ __forceinline__ F3 s_mul(float a, F3 b) { return { a * b.x, a * b.y, a * b.z }; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b)
+{
+	return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x };
+}
+__device__ __forceinline__ float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float mag2(F3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+// normalised(): a * (1/sqrt(a.a)) with IEEE sqrt and division.
+__device__ __forceinline__ F3 normalised(F3 a)
+{
+	const float r = 1.0f / sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+	return v_mul(a, r);
+}
+// vector_math.h:171-186
+__device__ inline F3 make_normal_vec(F3 dir, float phi)
+{
+	const float az = m_atan2(dir.y, dir.x);
+	const float sin_az = m_sin(az), cos_az = m_cos(az);
+	const F3 unit_v = { dir.z * cos_az, dir.z * sin_az, -sqrtf(dir.x * dir.x + dir.y * dir.y) };
+	const F3 unit_u = cross(unit_v, dir);
+	const float sin_phi = m_sin(phi), cos_phi = m_cos(phi);
+	return v_add(v_mul(unit_u, cos_phi), v_mul(unit_v, sin_phi));
+}
+// vector_math.h:194-206
+__device__ inline F3 make_unit_vec(float cos_theta, float phi)
+{
+	const float sin_phi = m_sin(phi), cos_phi = m_cos(phi);
+	cos_theta = clampr(cos_theta, -1.0f, 1.0f);
+	const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+	return { sin_theta * cos_phi, sin_theta * sin_phi, cos_theta };
+}
+
+// ------------------------------------------------------------------- RNG --
+// Counter-based Philox4x32-10 (Salmon et al., SC'11): key = run seed,
+// counter = (sid lo, sid hi, step, block of four draws).  A particle's draws
+// depend only on (seed, sid, step, draw index), so trajectories are
+// independent of scheduling and can be replayed.
+struct Rng {
+	uint32_t k0, k1, c0, c1, c2, idx;
+	uint32_t b0, b1, b2, b3;
+};
+
+__device__ __forceinline__ void philox4x32_10(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
+	uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+	for (int r = 0; r < 10; ++r) {
+		const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+		const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+		const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+		c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+		k0 += 0x9E3779B9u;
+		k1 += 0xBB67AE85u;
+	}
+}
+
+__device__ __forceinline__ void rng_start(Rng& r, uint32_t k0, uint32_t k1, uint64_t sid, uint32_t step)
+{
+	r.k0 = k0; r.k1 = k1;
+	r.c0 = (uint32_t)sid; r.c1 = (uint32_t)(sid >> 32); r.c2 = step;
+	r.idx = 0;
+}
+
+// Uniform in (0, 1]: ((w >> 8) + 1) * 2^-24, exact in float.
+__device__ __forceinline__ float rng_unit(Rng& r)
+{
+	const uint32_t i = r.idx & 3u;
+	if (i == 0) {
+		uint32_t a = r.c0, b = r.c1, c = r.c2, d = r.idx >> 2;
+		philox4x32_10(a, b, c, d, r.k0, r.k1);
+		r.b0 = a; r.b1 = b; r.b2 = c; r.b3 = d;
+	}
+	const uint32_t w = (i == 0) ? r.b0 : (i == 1) ? r.b1 : (i == 2) ? r.b2 : r.b3;
+	r.idx++;
+	return (float)((w >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+// Resume a stream at draw `idx` (re-running an event whose path draws were
+// taken by an earlier launch).
+__device__ __forceinline__ void rng_seek(Rng& r, uint32_t idx)
+{
+	const uint32_t blk = idx >> 2;
+	uint32_t a = r.c0, b = r.c1, c = r.c2, d = blk;
+	philox4x32_10(a, b, c, d, r.k0, r.k1);
+	r.b0 = a; r.b1 = b; r.b2 = c; r.b3 = d;
+	r.idx = idx;   // rng_unit recomputes the block if idx is a multiple of four
+}
+
+// common/util/random.h:56-84: phi = 2 pi U; uniform_vector draws phi first.
+__device__ __forceinline__ float rng_phi(Rng& r) { return (2.0f * PI_F) * rng_unit(r); }
+__device__ __forceinline__ F3 rng_uniform_vector(Rng& r)
+{
+	const float phi = rng_phi(r);
+	const float ct = 2.0f * rng_unit(r) - 1.0f;
+	return make_unit_vec(ct, phi);
+}
+
+// Stream id of a secondary electron: splitmix64 finaliser of the parent's
+// (sid, step), top bit set.
+__device__ __forceinline__ uint64_t se_sid(uint64_t parent_sid, uint32_t parent_step)
+{
+	uint64_t z = parent_sid * 0x9E3779B97F4A7C15ull + (uint64_t)parent_step + 1u;
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	z ^= z >> 31;
+	return z | 0x8000000000000000ull;
+}
+
+// ---------------------------------------------------------------- tables --
+struct Table1 { const float* d; int n; float xmin, xstep; };
+struct Table2 { const float* d; int nx, ny; float xmin, xstep, ymin, ystep; };
+struct Table3 { const float* d; int nx, ny, nz; float xmin, xstep, ymin, ystep, zmin, zstep; };
+
+// common/util/table_1D.inl:66-81
+__device__ __forceinline__ float t1_get(const Table1& t, float x)
+{
+	const float xi = (x - t.xmin) * t.xstep;
+	const int low = clampi(floorr(xi), 0, t.n - 2);
+	const float lv = t.d[low], hv = t.d[low + 1];
+	const float frac = xi - (float)low;
+	return (1.0f - frac) * lv + frac * hv;
+}
+
+// common/util/table_2D.inl:73-94 (row-major [x][y])
+__device__ __forceinline__ float t2_get(const Table2& t, float x, float y)
+{
+	const float xi = (x - t.xmin) * t.xstep;
+	const float yi = (y - t.ymin) * t.ystep;
+	const int lx = clampi(floorr(xi), 0, t.nx - 2);
+	const int ly = clampi(floorr(yi), 0, t.ny - 2);
+	const float* r0 = t.d + (size_t)lx * t.ny + ly;
+	const float* r1 = r0 + t.ny;
+	const float v00 = r0[0], v10 = r1[0], v01 = r0[1], v11 = r1[1];
+	const float fx = xi - (float)lx, fy = yi - (float)ly;
+	return (1.0f - fx) * (1.0f - fy) * v00
+		+ fx * (1.0f - fy) * v10
+		+ (1.0f - fx) * fy * v01
+		+ fx * fy * v11;
+}
+
+// common/util/table_2D.inl:96-114
+__device__ __forceinline__ float t2_get_rounddown(const Table2& t, float x, float y)
+{
+	const float xi = (x - t.xmin) * t.xstep;
+	const float yi = (y - t.ymin) * t.ystep;
+	const int lx = clampi(floorr(xi), 0, t.nx - 1);
+	const int ly = clampi(floorr(yi), 0, t.ny - 1);
+	return t.d[(size_t)lx * t.ny + ly];
+}
+__device__ __forceinline__ float t2_get_nearest(const Table2& t, float x, float y)
+{
+	const float xi = (x - t.xmin) * t.xstep;
+	const float yi = (y - t.ymin) * t.ystep;
+	const int nx = clampi(rintr(xi), 0, t.nx - 1);
+	const int ny = clampi(rintr(yi), 0, t.ny - 1);
+	return t.d[(size_t)nx * t.ny + ny];
+}
+
+// common/util/table_3D.inl:78-121 ([x][y][z])
+__device__ __forceinline__ float t3_get(const Table3& t, float x, float y, float z)
+{
+	const float xi = (x - t.xmin) * t.xstep;
+	const float yi = (y - t.ymin) * t.ystep;
+	const float zi = (z - t.zmin) * t.zstep;
+	const int lx = clampi(floorr(xi), 0, t.nx - 2);
+	const int ly = clampi(floorr(yi), 0, t.ny - 2);
+	const int lz = clampi(floorr(zi), 0, t.nz - 2);
+	const size_t sy = (size_t)t.nz, sx = (size_t)t.ny * t.nz;
+	const float* p = t.d + (size_t)lx * sx + (size_t)ly * sy + lz;
+	const float v000 = p[0], v001 = p[1];
+	const float v010 = p[sy], v011 = p[sy + 1];
+	const float v100 = p[sx], v101 = p[sx + 1];
+	const float v110 = p[sx + sy], v111 = p[sx + sy + 1];
+	const float fx = xi - (float)lx, fy = yi - (float)ly, fz = zi - (float)lz;
+	return (1.0f - fx) * (1.0f - fy) * (1.0f - fz) * v000
+		+ fx * (1.0f - fy) * (1.0f - fz) * v100
+		+ (1.0f - fx) * fy * (1.0f - fz) * v010
+		+ fx * fy * (1.0f - fz) * v110
+		+ (1.0f - fx) * (1.0f - fy) * fz * v001
+		+ fx * (1.0f - fy) * fz * v101
+		+ (1.0f - fx) * fy * fz * v011
+		+ fx * fy * fz * v111;
+}
+__device__ __forceinline__ float t3_get_rounddown(const Table3& t, float x, float y, float z)
+{
+	const float xi = (x - t.xmin) * t.xstep;
+	const float yi = (y - t.ymin) * t.ystep;
+	const float zi = (z - t.zmin) * t.zstep;
+	const int lx = clampi(floorr(xi), 0, t.nx - 1);
+	const int ly = clampi(floorr(yi), 0, t.ny - 1);
+	const int lz = clampi(floorr(zi), 0, t.nz - 1);
+	return t.d[((size_t)lx * t.ny + ly) * t.nz + lz];
+}
+
+// -------------------------------------------------------------- material --
+struct Material {
+	float barrier, phonon_loss, recoil_const, fermi, band_gap;
+	int inel_model;
+	Table1 el_imfp, in_imfp;
+	Table2 el_icdf, in_icdf, ki_ion;
+	Table3 pn_q, pn_ion;
+};
+
+// -------------------------------------------------------------- geometry --
+// Triangle as three float4: (r0.xyz, e1.x), (e1.yz, e2.xy), (e2.z, mat_in,
+// mat_out, 0) with the materials stored as int bits.
+struct Tri { F3 r0, e1, e2; int mat_in, mat_out; };
+
+__device__ __forceinline__ Tri load_tri(const float4* t, int i)
+{
+	const float4 a = t[3 * i], b = t[3 * i + 1], c = t[3 * i + 2];
+	Tri r;
+	r.r0 = { a.x, a.y, a.z };
+	r.e1 = { a.w, b.x, b.y };
+	r.e2 = { b.z, b.w, c.x };
+	r.mat_in = __float_as_int(c.y);
+	r.mat_out = __float_as_int(c.z);
+	return r;
+}
+
+struct Geometry {
+	const int* od;
+	const float4* tris;
+	F3 center, halfsize;
+	float max_extent;
+	int ntri;
+};
+
+// core/triangle.inl:7-28
+__device__ __forceinline__ float intersect_ray(const Tri& t, F3 start, F3 dir)
+{
+	const F3 pvec = cross(dir, t.e2);
+	const float det = dot(t.e1, pvec);
+	if (fabsf(det) < EPS)
+		return -1.0f;
+	const F3 s = v_sub(start, t.r0);
+	const float u = dot(s, pvec) / det;
+	if ((u < -EPS) || (u > 1.0f + EPS))
+		return -1.0f;
+	const F3 qvec = cross(s, t.e1);
+	const float v = dot(dir, qvec) / det;
+	if ((v < -EPS) || (u + v > 1.0f + EPS))
+		return -1.0f;
+	const float tt = dot(t.e2, qvec) / det;
+	if (tt < 0.0f)
+		return -1.0f;
+	return tt;
+}
+
+// geometry/octree.inl:24-30
+__device__ __forceinline__ bool in_domain(const Geometry& g, F3 p)
+{
+	const F3 c = g.center, h = g.halfsize;
+	return (p.x > c.x - h.x) && (p.x < c.x + h.x)
+		&& (p.y > c.y - h.y) && (p.y < c.y + h.y)
+		&& (p.z > c.z - h.z) && (p.z < c.z + h.z);
+}
+
+// geometry/octree.inl:32-192: walk leaf cells by location code; returns the
+// hit triangle (or -1) and the distance.
+__device__ inline int propagate(const Geometry& g, F3 start, F3 dir, float distance,
+	int ignore_tri, int ignore_mat, float& out_dist)
+{
+	const int* __restrict__ od = g.od;
+	const F3 hs = g.halfsize;
+	F3 cur = start;
+	float to_go = distance;
+	uint64_t location = 1;
+	uint32_t cells = 0;
+	do {
+		float ax = g.center.x, ay = g.center.y, az = g.center.z, aw = 1.0f;
+		int index = 0;
+		for (int i = 60 - __clzll((long long)location); i >= 0; i -= 3) {
+			const unsigned oct = (unsigned)(location >> i) & 7u;
+			index = od[index + (int)oct];
+			ax += aw * hs.x * ((float)(oct & 1u) - 0.5f);
+			ay += aw * hs.y * ((float)((oct & 2u) >> 1) - 0.5f);
+			az += aw * hs.z * ((float)((oct & 4u) >> 2) - 0.5f);
+			aw *= 0.5f;
+		}
+		while (index >= 0) {
+			unsigned oct = 0;
+			oct ^= (cur.x > ax) ? 1u : 0u;
+			oct ^= (cur.y > ay) ? 2u : 0u;
+			oct ^= (cur.z > az) ? 4u : 0u;
+			location = (location << 3) | oct;
+			index = od[index + (int)oct];
+			ax += aw * hs.x * ((float)(oct & 1u) - 0.5f);
+			ay += aw * hs.y * ((float)((oct & 2u) >> 1) - 0.5f);
+			az += aw * hs.z * ((float)((oct & 4u) >> 2) - 0.5f);
+			aw *= 0.5f;
+		}
+		index = -index;
+		++cells;
+
+		float isect;
+		int target;
+		{
+			const float hx = aw * hs.x, hy = aw * hs.y, hz = aw * hs.z;
+			const float tx = (ax + copysignf(hx + EPS, dir.x) - cur.x) / dir.x;
+			const float ty = (ay + copysignf(hy + EPS, dir.y) - cur.y) / dir.y;
+			const float tz = (az + copysignf(hz + EPS, dir.z) - cur.z) / dir.z;
+			if ((tx < ty) && (tx < tz)) { isect = tx; target = -1; }
+			else if ((ty < tx) && (ty < tz)) { isect = ty; target = -2; }
+			else { isect = tz; target = -4; }
+		}
+
+		for (;;) {
+			const int ti = od[index++];
+			if (ti < 0)
+				break;
+			if (ti == ignore_tri)
+				continue;
+			const Tri tri = load_tri(g.tris, ti);
+			const int mat_out = (dot(cross(tri.e1, tri.e2), dir) < 0.0f) ? tri.mat_in : tri.mat_out;
+			if (mat_out == ignore_mat)
+				continue;
+			const float t = intersect_ray(tri, cur, dir);
+			if ((t > 0.0f) && (t <= isect + EPS)) {
+				isect = t;
+				target = ti;
+			}
+		}
+
+		if (isect >= to_go) {
+			out_dist = distance;
+			return -1;
+		} else if (target >= 0) {
+			out_dist = distance - to_go + isect;
+			return target;
+		}
+		to_go -= isect;
+		cur = v_add(cur, v_mul(dir, isect));
+
+		const unsigned mask = (unsigned)(-target);
+		unsigned value;
+		if (mask == 1u) value = (dir.x >= 0.0f) ? 0u : 1u;
+		else if (mask == 2u) value = (dir.y >= 0.0f) ? 0u : 2u;
+		else value = (dir.z >= 0.0f) ? 0u : 4u;
+		while (location > 1) {
+			if ((location & (uint64_t)mask) == (uint64_t)value) {
+				location ^= (uint64_t)mask;
+				break;
+			}
+			location >>= 3;
+		}
+	} while (location > 1 && cells < MAX_CELLS);
+	out_dist = distance;
+	return -1;
+}
+
+// -------------------------------------------------------------- physics --
+struct Particle { F3 pos, dir; float E; };
+
+// physics/full_penn.h:50-57 (same body in elastic.h:54-62)
+__device__ __forceinline__ float mech_sample_path(const Table1& t, float E, Rng& rng)
+{
+	const float imfp = m_exp(t1_get(t, m_log(E)));
+	const float tau = 1.0f / imfp;
+	return -tau * m_log(rng_unit(rng));
+}
+
+// physics/kieft/elastic.h:67-115
+__device__ inline void elastic_execute(const Material& m, uint32_t fl, Particle& p, Rng& rng)
+{
+	Particle tp = p;
+	float cos_theta, sin_theta;
+	{
+		const float x = m_log(p.E);
+		const float y = rng_unit(rng);
+		cos_theta = clampr(t2_get(m.el_icdf, x, y), -1.0f, 1.0f);
+		sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+	}
+	tp.dir = normalised(tp.dir);
+	const F3 normal_dir = normalised(make_normal_vec(tp.dir, rng_phi(rng)));
+	tp.dir = v_add(v_mul(tp.dir, cos_theta), v_mul(normal_dir, sin_theta));
+	if (tp.E < 200.0f) {
+		if (fl & PH_EL_PHONON)
+			tp.E = tp.E - fmin_std(50e-3f, m.phonon_loss);
+	} else {
+		if (fl & PH_EL_RECOIL)
+			tp.E = tp.E - m.recoil_const * (1.0f - cos_theta) * tp.E;
+	}
+	p = tp;
+}
+
+// physics/full_penn.h:62-288
+__device__ inline bool penn_execute(const Material& m, uint32_t fl, Particle& p, Rng& rng, Particle& se)
+{
+	Particle tp = p;
+	float omega;
+	{
+		const float x = m_log(tp.E);
+		const float y = rng_unit(rng);
+		omega = m_exp(t2_get(m.in_icdf, x, y));
+	}
+	float binding;
+	{
+		const float x = m_log(tp.E);
+		const float y = m_log(omega / tp.E);
+		const float z = rng_unit(rng);
+		binding = t3_get_rounddown(m.pn_ion, x, y, z);
+	}
+	tp.dir = normalised(tp.dir);
+
+	if (binding <= 0.0f) {
+		if (omega > m.band_gap) {
+			float q;
+			{
+				const float x = m_log(tp.E);
+				const float y = omega / tp.E;
+				const float z = rng_unit(rng);
+				q = t3_get(m.pn_q, x, y, z);
+			}
+			const F3 prim_normal_dir = normalised(make_normal_vec(tp.dir, rng_phi(rng)));
+			F3 q_dir, q_normal_dir;
+			{
+				const float c = clampr((omega + q * q) / (2.0f * sqrtf(tp.E) * q), -1.0f, 1.0f);
+				const float s = sqrtf(1.0f - c * c);
+				q_dir = normalised(v_add(v_mul(tp.dir, c), v_mul(prim_normal_dir, s)));
+				q_normal_dir = normalised(make_normal_vec(q_dir, rng_phi(rng)));
+			}
+			const float cpf = clampr((2.0f * tp.E - omega - q * q) / (2.0f * sqrtf(tp.E * (tp.E - omega))), -1.0f, 1.0f);
+			const float spf = sqrtf(1.0f - cpf * cpf);
+			tp.E = tp.E - omega;
+			tp.dir = v_add(v_mul(tp.dir, cpf), v_mul(prim_normal_dir, spf));
+			p = tp;
+			if (!(fl & PH_PN_SE))
+				return false;
+			Particle sp;
+			sp.pos = tp.pos;
+			const float k_fermi = sqrtf(m.fermi);
+			const float k_fermi_omega = sqrtf(m.fermi + omega);
+			if (q < k_fermi_omega - k_fermi) {
+				// plasmon decay (:141-194)
+				const float f = m.fermi / omega;
+				float x;
+				if (f < 0.1f) {
+					const float U = m_cbrt(rng_unit(rng));
+					x = f * U * U;
+				} else {
+					const float lower = fmax_std(0.0f, f - 1.0f);
+					const float upper = f;
+					const float d1 = upper - lower;
+					const float d2 = upper * upper - lower * lower;
+					const float alpha = d2 / (d1 + d2);
+					int rounds = 0;
+					do {
+						const float U1 = rng_unit(rng);
+						const float U2 = rng_unit(rng);
+						x = (U1 < alpha) ? sqrtf(d2 * U2 + lower * lower) : d1 * U2 + lower;
+					} while (rng_unit(rng) > sqrtf(x * x + x) / (x + 0.5f) && ++rounds < MAX_REJECT);
+				}
+				sp.E = x * omega + omega;
+				sp.dir = rng_uniform_vector(rng);
+			} else if (q < k_fermi_omega + k_fermi) {
+				// single excitation (:195-222)
+				const float ki_parr = (omega - q * q) / (2.0f * q);
+				const float kf_parr = ki_parr + q;
+				const float ki_perp_min_sq = fmax_std(0.0f, m.fermi - kf_parr * kf_parr);
+				const float ki_perp_max_sq = m.fermi - ki_parr * ki_parr;
+				const float ki_perp_sq = fmax_std(0.0f,
+					ki_perp_min_sq + (ki_perp_max_sq - ki_perp_min_sq) * rng_unit(rng));
+				const F3 ki = v_add(s_mul(ki_parr, q_dir), s_mul(sqrtf(ki_perp_sq), q_normal_dir));
+				sp.E = mag2(ki) + omega;
+				sp.dir = v_add(ki, s_mul(q, q_dir));
+			} else {
+				return false;
+			}
+			se = sp;
+			return true;
+		}
+		tp.E = tp.E - omega;   // sub-band-gap loss (:234-242)
+		p = tp;
+		return false;
+	}
+
+	// inner-shell excitation (:245-287)
+	const F3 normal_dir = normalised(make_normal_vec(tp.dir, rng_phi(rng)));
+	const float K = tp.E - m.fermi + 2.0f * binding;
+	const float dK = binding + omega;
+	const float cos_theta = sqrtf(dK / K);
+	const float sin_theta = sqrtf(1.0f - saturater(cos_theta * cos_theta));
+	F3 sdir = v_add(v_mul(tp.dir, cos_theta), v_mul(normal_dir, sin_theta));
+	sdir = normalised(sdir);
+	sdir = v_add(sdir, s_mul(sqrtf(binding / dK), rng_uniform_vector(rng)));
+	sdir = normalised(sdir);
+	bool made = false;
+	if (fl & PH_PN_SE) {
+		se.E = m.fermi + omega - binding;
+		se.pos = tp.pos;
+		se.dir = sdir;
+		made = true;
+	}
+	tp.E = tp.E - omega;
+	tp.dir = v_sub(tp.dir, s_mul(cos_theta, sdir));
+	p = tp;
+	return made;
+}
+
+// physics/kieft/inelastic.h:68-206
+__device__ inline bool kieft_execute(const Material& m, uint32_t fl, Particle& p, Rng& rng, Particle& se)
+{
+	Particle tp = p;
+	float omega0;
+	{
+		const float x = m_log(tp.E);
+		const float y = rng_unit(rng);
+		omega0 = m_exp(t2_get(m.in_icdf, x, y));
+	}
+	float binding;
+	{
+		const float x = m_log(omega0);
+		const float y = rng_unit(rng);
+		binding = t2_get_rounddown(m.ki_ion, x, y);
+	}
+	float omega;
+	{
+		const float omega_max = 0.5f * (tp.E + omega0 - m.fermi);
+		float omega_min = omega0;
+		float w0 = fmin_std(omega0 - 1.0f, fmax_std(0.0f, binding) - m.fermi);
+		if (tp.E > 2.0f * omega0) {
+			omega_min = 0.5f * (tp.E + omega0 - sqrtf(tp.E * (tp.E - 2.0f * omega0)));
+			w0 = omega0;
+		}
+		const float U = rng_unit(rng);
+		if ((w0 > 0.0f) && (omega_min > w0) && (omega_min < omega_max)) {
+			const float f_min = 1.0f / w0 * m_log((omega_min - w0) / omega_min);
+			const float f_max = 1.0f / w0 * m_log((omega_max - w0) / omega_max);
+			omega = -w0 / m_expm1(w0 * (f_min * (1.0f - U) + f_max * U));
+		} else {
+			omega = omega0 * omega_max / (omega0 * (1.0f - U) + omega_max * U);
+		}
+	}
+	if (binding < 0.0f) {
+		if (m.band_gap < 0.0f) {
+			// metals: no special case in the reference
+		} else if (omega0 > m.band_gap) {
+			binding = m.band_gap;
+		} else {
+			if (fl & PH_KI_OPTICAL) {
+				tp.E = tp.E - omega0;
+				p = tp;
+			}
+			return false;
+		}
+	}
+	binding = fmax_std(0.0f, binding);
+	tp.dir = normalised(tp.dir);
+	const F3 normal_dir = normalised(make_normal_vec(tp.dir, rng_phi(rng)));
+	const float K = tp.E - m.fermi + 2.0f * binding;
+	const float dK = binding + omega;
+	const float cos_theta = sqrtf(dK / K);
+	const float sin_theta = sqrtf(1.0f - saturater(cos_theta * cos_theta));
+	F3 sdir = v_add(v_mul(tp.dir, cos_theta), v_mul(normal_dir, sin_theta));
+	if (fl & PH_KI_INST) {
+		sdir = normalised(sdir);
+		sdir = v_add(sdir, s_mul(sqrtf(binding / dK), rng_uniform_vector(rng)));
+	}
+	sdir = normalised(sdir);
+	bool made = false;
+	if (fl & PH_KI_SE) {
+		se.E = m.fermi + omega - binding;
+		se.pos = tp.pos;
+		se.dir = sdir;
+		made = true;
+	}
+	tp.E = tp.E - omega;
+	if (fl & PH_KI_MOMCONS)
+		tp.dir = v_sub(tp.dir, s_mul(cos_theta, sdir));
+	p = tp;
+	return made;
+}
+
+// physics/boundary_intersect.h:38-155
+__device__ inline int boundary_execute(const Material* mats, uint32_t fl, const Tri& tri,
+	Particle& p, int& mat, Rng& rng)
+{
+	Particle tp = p;
+	const F3 nd = normalised(tp.dir);
+	const F3 n = normalised(cross(tri.e1, tri.e2));
+	const float cos_theta = dot(n, nd);
+	int mi, mo;
+	if (cos_theta > 0.0f) { mi = tri.mat_in; mo = tri.mat_out; }
+	else { mi = tri.mat_out; mo = tri.mat_in; }
+	switch (mo) {
+	case MAT_DETECTOR:
+		return DETECTED;
+	case MAT_DETECTOR_LT50:
+		return (tp.E < 50.0f) ? DETECTED : ALIVE;
+	case MAT_DETECTOR_GE50:
+		return (tp.E >= 50.0f) ? DETECTED : ALIVE;
+	case MAT_TERMINATOR:
+		return TERMINATED;
+	case MAT_MIRROR:
+		tp.dir = v_sub(nd, v_mul(s_mul(2.0f, n), cos_theta));
+		p = tp;
+		return ALIVE;
+	case MAT_NOP:
+		return ALIVE;
+	default:
+		break;
+	}
+	float dU = 0.0f;
+	if (mi >= 0) dU = dU - mats[mi].barrier;
+	if (mo >= 0) dU = dU + mats[mo].barrier;
+	if (tp.E * cos_theta * cos_theta + dU > 0.0f) {
+		const float s = sqrtf(1.0f + dU / (tp.E * cos_theta * cos_theta));
+		const float T = (fl & PH_BI_QUANTUM) ? 4.0f * s / ((1.0f + s) * (1.0f + s)) : 1.0f;
+		if (rng_unit(rng) < T) {
+			if (fl & PH_BI_REFRACT)
+				tp.dir = v_add(v_sub(nd, v_mul(n, cos_theta)), v_mul(s_mul(s, n), cos_theta));
+			tp.E = tp.E + dU;
+			mat = mo;
+			p = tp;
+			return ALIVE;
+		}
+	}
+	if (fl & PH_BI_ABSORB) {
+		if (dU < 0.0f && rng_unit(rng) < m_exp(1.0f + 0.5f * tp.E / dU))
+			return TERMINATED;
+	}
+	tp.dir = v_sub(nd, v_mul(s_mul(2.0f, n), cos_theta));
+	p = tp;
+	return ALIVE;
+}
+
+// --------------------------------------------------------------- a step --
+struct State {
+	Particle p;
+	int mat, last_tri;
+	uint32_t tag, step, flags;
+	uint64_t sid;
+};
+
+struct World {
+	Geometry g;
+	const Material* mats;
+	int nmats;
+	uint32_t flags;
+	float threshold;
+	uint32_t key0, key1;
+};
+
+// init (drivers/gpu/gpu_driver.inl:403-480): domain and energy checks, path
+// sampling, propagate, move; stores the normalised direction.  Returns ALIVE
+// or TERMINATED and the event; `hit` is the intersected triangle.
+__device__ inline int step_init(const World& W, State& s, Rng& rng, int& event, int& hit)
+{
+	event = EV_NONE;
+	hit = -1;
+	if (!in_domain(W.g, s.p.pos))
+		return TERMINATED;
+	int type = 0;
+	float distance = W.g.max_extent;
+	if (s.mat >= 0) {
+		const Material& m = W.mats[s.mat];
+		if (s.p.E < m.barrier + W.threshold)
+			return TERMINATED;
+		// core/scatter_list.h:101-107 with the reverse visiting order of
+		// common/tuple.h:114-124: elastic first, strictly shorter wins.
+		float dist = __builtin_inff();
+		const float pe = mech_sample_path(m.el_imfp, s.p.E, rng);
+		if (pe < dist) { type = 2; dist = pe; }
+		const float pi = mech_sample_path(m.in_imfp, s.p.E, rng);
+		if (pi < dist) { type = 1; dist = pi; }
+		distance = dist;
+	}
+	const F3 nd = normalised(s.p.dir);
+	float hd;
+	hit = propagate(W.g, s.p.pos, nd, distance, s.last_tri, s.mat, hd);
+	s.p.dir = nd;
+	if (hit < 0) {
+		s.p.pos = v_add(s.p.pos, v_mul(nd, distance));
+		event = type;
+	} else {
+		s.p.pos = v_add(s.p.pos, v_mul(nd, hd));
+		s.last_tri = hit;
+		event = EV_INTERSECT;
+	}
+	return ALIVE;
+}
+
+// The event chosen by init (gpu_driver.inl:482-566).  For inelastic events
+// the new parent state and secondary are computed into `s`/`se`; the caller
+// commits them.
+__device__ inline int step_event(const World& W, State& s, Rng& rng, int event, int hit,
+	Particle& se, bool& se_made)
+{
+	se_made = false;
+	if (event == EV_INTERSECT) {
+		const Tri tri = load_tri(W.g.tris, hit);
+		return boundary_execute(W.mats, W.flags, tri, s.p, s.mat, rng);
+	}
+	if (event == EV_NONE)
+		return ALIVE;
+	s.last_tri = -1;
+	const Material& m = W.mats[s.mat];
+	if (event == EV_ELASTIC) {
+		elastic_execute(m, W.flags, s.p, rng);
+		return ALIVE;
+	}
+	se_made = (m.inel_model == 0) ? penn_execute(m, W.flags, s.p, rng, se)
+		: kieft_execute(m, W.flags, s.p, rng, se);
+	return ALIVE;
+}
+
+}} // namespace nbl::dev

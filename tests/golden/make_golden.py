@@ -1,0 +1,230 @@
+"""Write the golden vectors that pin the oracle, from the reference's own CPU
+code built from its sources by oracle/ref/Makefile (binaries in
+oracle/_ref/).  Run in the container that holds the reference:
+
+    make -C oracle/ref && python tests/golden/make_golden.py
+
+Inputs are generated here (synthetic geometries, materials written as .mat
+files by nebula_amd's HDF5 writer, seeded rays / states / beams), passed to
+the harness, and stored together with the outputs as compressed .npz
+fixtures.  Large prepared tables are stored as SHA-256 digests plus a sampled
+subset.
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from nebula_amd import api as A  # noqa: E402
+from nebula_amd import workloads as W  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "harness")
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+RAY = np.dtype([("pos", "<f4", 3), ("dir", "<f4", 3), ("dist", "<f4"), ("ignore_tri", "<i4"),
+                ("ignore_mat", "<i4"), ("tri", "<i4")])
+ST = np.dtype([("pos", "<f4", 3), ("dir", "<f4", 3), ("E", "<f4"), ("mat", "<i4"), ("last_tri", "<i4")])
+PART = np.dtype([("pos", "<f4", 3), ("dir", "<f4", 3), ("E", "<f4")])
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def run(*args):
+    subprocess.run([HARNESS, *map(str, args)], check=True)
+
+
+def tri_file(path, mesh: W.Mesh):
+    raw = np.zeros(mesh.n, dtype=[("in", "<i4"), ("out", "<i4"), ("v", "<f4", 9)])
+    raw["in"], raw["out"], raw["v"] = mesh.mat_in, mesh.mat_out, mesh.vertices()
+    raw.tofile(path)
+
+
+def small_grating():
+    return W.grating_geometry(n_lines=2, length=64.0, dy=4.0, nz=3, substrate_depth=60.0, seed=3)
+
+
+def rays_for(center, half, n, seed, ntri):
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, RAY)
+    r["pos"] = center + (rng.random((n, 3)) * 2 - 1) * half * 0.98
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = d.astype(np.float32)
+    s = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]).astype(np.float32)
+    s = (s + d[:, 2] * d[:, 2]).astype(np.float32)
+    r["dir"] = (d * (np.float32(1) / np.sqrt(s))[:, None]).astype(np.float32)
+    r["dist"] = rng.exponential(40.0, n) + 0.05
+    r["ignore_tri"] = np.where(rng.random(n) < 0.3, rng.integers(0, ntri, n), -1)
+    r["ignore_mat"] = rng.choice([-123, 0, 1], n)
+    r["tri"] = rng.integers(0, ntri, n)
+    return r
+
+
+def states_for(n, seed, zlo=-55.0, zhi=-0.01):
+    rng = np.random.default_rng(seed)
+    s = np.zeros(n, ST)
+    s["pos"][:, 0] = rng.uniform(-30, 30, n)
+    s["pos"][:, 1] = rng.uniform(-30, 30, n)
+    s["pos"][:, 2] = np.where(rng.random(n) < 0.3, -rng.uniform(0.01, 2.0, n), rng.uniform(zlo, zhi, n))
+    d = rng.standard_normal((n, 3))
+    s["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    s["E"] = np.exp(rng.uniform(np.log(14.0), np.log(5000.0), n))
+    s["mat"] = 0
+    s["last_tri"] = -1
+    return s
+
+
+def main():
+    if not os.path.exists(HARNESS):
+        sys.exit("oracle/_ref/harness missing: run `make -C oracle/ref` first")
+    tmp = tempfile.mkdtemp()
+    fx = {}
+    # ------------------------------------------------ geometries + octrees
+    meshes = {"slab": W.slab_geometry(), "grating_small": small_grating(),
+              "grating": W.grating_geometry(dy=3.2)}
+    for name, mesh in meshes.items():
+        tf = os.path.join(tmp, f"{name}.tris")
+        tri_file(tf, mesh)
+        of = os.path.join(tmp, f"{name}.oct")
+        run("octree", tf, of)
+        b = open(of, "rb").read()
+        n_od = np.frombuffer(b, "<i8", 1, 0)[0]
+        od = np.frombuffer(b, "<i4", n_od, 8)
+        off = 8 + 4 * n_od
+        n_t = np.frombuffer(b, "<i8", 1, off)[0]
+        tris = np.frombuffer(b, A.TRI_DTYPE, n_t, off + 8)
+        off2 = off + 8 + 44 * n_t
+        ch = np.frombuffer(b, "<f4", 6, off2)
+        fx[f"{name}_mesh_sha"] = sha(mesh.vertices()) + sha(mesh.mat_in) + sha(mesh.mat_out)
+        fx[f"{name}_octree_sha"] = sha(od)
+        fx[f"{name}_sorted_sha"] = sha(tris)
+        fx[f"{name}_n_octree"] = n_od
+        fx[f"{name}_center_halfsize"] = ch.copy()
+        if name != "grating":
+            fx[f"{name}_octree"] = od.copy()
+            fx[f"{name}_sorted"] = tris.copy()
+        if name in ("slab", "grating_small"):
+            rays = rays_for(ch[:3], ch[3:], 4000, seed=21 if name == "slab" else 22, ntri=n_t)
+            rf = os.path.join(tmp, f"{name}.rays")
+            rays.tofile(rf)
+            pf = os.path.join(tmp, f"{name}.prop")
+            run("propagate", tf, rf, pf)
+            pr = np.fromfile(pf, np.uint8)
+            res = pr[:-4].view([("tri", "<i4"), ("dist", "<f4")])
+            fx[f"{name}_rays"] = rays
+            fx[f"{name}_prop_tri"] = res["tri"].copy()
+            fx[f"{name}_prop_dist"] = res["dist"].copy()
+            fx[f"{name}_max_extent"] = pr[-4:].view("<f4").copy()
+            # triangle tests against the unsorted input triangles
+            itf = os.path.join(tmp, f"{name}.isect")
+            run("intersect", tf, rf, itf)
+            fx[f"{name}_isect_t"] = np.fromfile(itf, "<f4")
+            print(f"{name}: {n_t} triangles, {n_od} octree ints, "
+                  f"{np.mean(res['tri'] >= 0):.3f} rays hit")
+    # ----------------------------------------------------------- materials
+    for model in ("penn", "kieft"):
+        for mname, spec in (("si", W.SI), ("pmma", W.PMMA)):
+            raw = W.make_material(spec, model)
+            mf = os.path.join(tmp, f"{mname}_{model}.mat")
+            A.write_mat_file(mf, raw)
+            df = os.path.join(tmp, f"{mname}_{model}.dump")
+            run("material", mf, model, df)
+            b = open(df, "rb").read()
+            scal = np.frombuffer(b, "<f4", 6, 0)
+            off = 24
+            tables = {}
+            names = ["el_imfp", "el_icdf", "in_imfp", "in_icdf"] + (["pn_q", "pn_ion"] if model == "penn" else ["ki_ion"])
+            for tn in names:
+                hdr = np.frombuffer(b, "<i4", 4, off)
+                sc = np.frombuffer(b, "<f4", 6, off + 16)
+                dims = [int(x) for x in hdr[1:1 + hdr[0]]]
+                cnt = int(np.prod(dims))
+                data = np.frombuffer(b, "<f4", cnt, off + 40).reshape(dims)
+                off += 40 + 4 * cnt
+                tables[tn] = (data, sc)
+                key = f"mat_{mname}_{model}_{tn}"
+                fx[key + "_sha"] = sha(data)
+                fx[key + "_scale"] = sc.copy()
+                fx[key + "_dims"] = np.array(dims)
+                flat = data.ravel()
+                idx = np.random.default_rng(1).integers(0, len(flat), min(512, len(flat)))
+                fx[key + "_sample_idx"] = idx
+                fx[key + "_sample"] = flat[idx].copy()
+            fx[f"mat_{mname}_{model}_scalars"] = scal.copy()
+            fx[f"mat_{mname}_{model}_raw_sha"] = "".join(sha(d.data) for d in raw.datasets.values())
+            # table probes
+            rng = np.random.default_rng(5)
+            pts = np.zeros(3000, [("x", "<f4"), ("y", "<f4"), ("z", "<f4")])
+            pts["x"] = rng.uniform(np.log(5.0), np.log(80e3), 3000)
+            pts["y"] = rng.uniform(-0.05, 1.05, 3000) if model == "kieft" else np.where(
+                rng.random(3000) < 0.5, rng.uniform(-0.05, 1.05, 3000), rng.uniform(np.log(1e-5), np.log(2.0), 3000))
+            pts["z"] = rng.uniform(-0.05, 1.05, 3000)
+            ptf = os.path.join(tmp, "pts.bin")
+            pts.tofile(ptf)
+            prf = os.path.join(tmp, "probe.bin")
+            run("probe", mf, model, ptf, prf)
+            k = 9 if model == "penn" else 8
+            fx[f"probe_{mname}_{model}_pts"] = pts
+            fx[f"probe_{mname}_{model}_out"] = np.fromfile(prf, "<f4").reshape(-1, k)
+    # ------------------------------------------------------- steps + runs
+    small = meshes["grating_small"]
+    tf = os.path.join(tmp, "grating_small.tris")
+    for model in ("penn", "kieft"):
+        mats = [os.path.join(tmp, f"si_{model}.mat"), os.path.join(tmp, f"pmma_{model}.mat")]
+        st = states_for(3000, seed=31 if model == "penn" else 32)
+        stf = os.path.join(tmp, "states.bin")
+        st.tofile(stf)
+        of = os.path.join(tmp, "steps.out")
+        run("step", tf, model, 1234, stf, of, *mats)
+        raw = np.fromfile(of, np.uint8)
+        recs = []
+        off = 0
+        head = np.dtype([("status", "<i4"), ("next", "<i4"), ("mat", "<i4"), ("last_tri", "<i4"),
+                         ("p", PART), ("nse", "<i4")])
+        se_dt = np.dtype([("p", PART), ("mat", "<i4")])
+        out = np.zeros(len(st), [("status", "<i4"), ("next", "<i4"), ("mat", "<i4"), ("last_tri", "<i4"),
+                                 ("p", PART), ("nse", "<i4"), ("se", PART), ("se_mat", "<i4")])
+        for i in range(len(st)):
+            h = raw[off:off + head.itemsize].view(head)[0]
+            off += head.itemsize
+            for f in head.names:
+                out[i][f] = h[f]
+            if h["nse"]:
+                s = raw[off:off + se_dt.itemsize].view(se_dt)[0]
+                off += se_dt.itemsize * h["nse"]
+                out[i]["se"] = s["p"]
+                out[i]["se_mat"] = s["mat"]
+        assert off == len(raw)
+        fx[f"step_{model}_states"] = st
+        fx[f"step_{model}_out"] = out
+        print(f"step {model}: status {np.bincount(out['status'])}, se {out['nse'].sum()}")
+        # whole cascades, one mt19937 stream
+        pri = W.raster_scan(4, 4, 8, 500.0, -20, 20, -20, 20, 36.0, seed=9)
+        pf = os.path.join(tmp, "pri.bin")
+        W.write_pri(pf, pri)
+        sf = os.path.join(tmp, "sim.out")
+        run("simulate", tf, model, 77, pf, sf, *mats)
+        b = np.fromfile(sf, np.uint8)
+        n = int(b[-8:].view("<i8")[0])
+        det = b[:-8].view([("p", PART), ("tag", "<u4")])
+        assert len(det) == n
+        fx[f"sim_{model}_pri"] = pri
+        fx[f"sim_{model}_det"] = det.copy()
+        print(f"simulate {model}: {len(pri)} primaries, {n} detected")
+    np.savez_compressed(os.path.join(OUT, "reference_vectors.npz"), **fx)
+    print("wrote", os.path.join(OUT, "reference_vectors.npz"),
+          os.path.getsize(os.path.join(OUT, "reference_vectors.npz")), "bytes")
+
+
+if __name__ == "__main__":
+    main()
