@@ -44,6 +44,9 @@ struct Ctrl {
 	uint32_t refill_n, refill_dst;
 	uint32_t target;        // particles in flight to aim for
 	uint32_t det_ctr;       // detected records written (may overshoot det_cap)
+	uint32_t spill_n;       // secondaries parked on the spill stack
+	uint32_t spill_ctr;     // spill stack top during a k_step (may overshoot)
+	uint32_t spill_take, spill_src;  // refill from the spill stack
 	uint32_t pad0;
 	uint64_t refill_src;
 	uint64_t queue_n, queue_next;
@@ -53,10 +56,12 @@ struct Ctrl {
 struct StepArgs {
 	World W;
 	Bufs buf[2];
+	Bufs spill;
 	Ctrl* ctrl;
 	nbl_detected* det;
 	uint32_t det_cap;
 	uint32_t slots;
+	uint32_t spill_cap;
 	uint32_t K;
 };
 
@@ -106,7 +111,9 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t* ctr, bool want)
 
 // ------------------------------------------------------------- kernels --
 // Adopt the previous launch's output and plan the refill from the queue.
-__global__ void k_ctrl(Ctrl* c, uint32_t slots)
+// Secondaries that found no room in the particle buffer wait on the spill
+// stack; they are taken back (newest first) before any new primary.
+__global__ void k_ctrl(Ctrl* c, uint32_t slots, uint32_t spill_cap)
 {
 	if (threadIdx.x != 0 || blockIdx.x != 0)
 		return;
@@ -114,38 +121,49 @@ __global__ void k_ctrl(Ctrl* c, uint32_t slots)
 		c->buf_in ^= 1u;
 		c->n_surv = c->surv_ctr;
 		c->n_se = min(c->se_ctr, c->se_limit);
+		c->spill_n = min(c->spill_ctr, spill_cap);
 		c->stepped = 0;
 	}
 	const uint32_t active = c->n_surv + c->n_se;
 	uint32_t room = c->target > active ? c->target - active : 0u;
 	room = min(room, slots - active);
+	const uint32_t take = min(c->spill_n, room);
+	room -= take;
 	const uint64_t left = c->queue_n - c->queue_next;
 	const uint32_t m = (uint32_t)min<uint64_t>(left, (uint64_t)room);
+	c->spill_take = take;
+	c->spill_src = c->spill_n - take;
+	c->spill_n -= take;
 	c->refill_src = c->queue_next;
 	c->refill_dst = c->n_surv;
-	c->refill_n = m;
+	c->refill_n = take + m;
 	c->queue_next += m;
-	c->n_surv += m;
+	c->n_surv += take + m;
 	c->se_limit = slots - (c->n_surv + c->n_se);
 	c->surv_ctr = 0;
 	c->se_ctr = 0;
+	c->spill_ctr = c->spill_n;
 	c->stepped = 1;
 }
 
 // Move planned primaries from the queue into the particle buffer
 // (gpu_driver.inl:367-389 / gpu_particle_manager.inl:342-353: vacuum
 // material, no last triangle).
-__global__ __launch_bounds__(256) void k_refill(Ctrl* c, Bufs b0, Bufs b1,
+__global__ __launch_bounds__(256) void k_refill(Ctrl* c, Bufs b0, Bufs b1, Bufs spill,
 	const nbl_particle28* __restrict__ q, const uint32_t* __restrict__ qtag,
 	const uint64_t* __restrict__ qsid)
 {
 	const uint32_t n = c->refill_n;
-	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
+	const uint32_t stride = gridDim.x * blockDim.x;
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
 	const Bufs& b = c->buf_in ? b1 : b0;
-	const uint64_t src = c->refill_src + i;
 	const uint32_t dst = c->refill_dst + i;
+	const uint32_t take = c->spill_take;
+	if (i < take) {
+		store_state(b, dst, load_state(spill, c->spill_src + i));
+		continue;
+	}
+	const uint64_t src = c->refill_src + (i - take);
 	const nbl_particle28 p = q[src];
 	State s;
 	s.p.pos = { p.pos[0], p.pos[1], p.pos[2] };
@@ -158,6 +176,7 @@ __global__ __launch_bounds__(256) void k_refill(Ctrl* c, Bufs b0, Bufs b1,
 	s.flags = 0;
 	s.sid = qsid[src];
 	store_state(b, dst, s);
+	}
 }
 
 __global__ __launch_bounds__(256) void k_step(StepArgs a)
@@ -202,18 +221,23 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
 			bool stop = false;
 			const uint32_t sslot = wave_alloc(&C->se_ctr, se_made);
 			if (se_made) {
-				if (sslot < se_limit) {
-					State ns;
-					ns.p = se;
-					ns.mat = s.mat;
-					ns.last_tri = -1;
-					ns.tag = s.tag;
-					ns.step = 0;
-					ns.flags = 0;
-					ns.sid = se_sid(s.sid, s.step);
+				State ns;
+				ns.p = se;
+				ns.mat = s.mat;
+				ns.last_tri = -1;
+				ns.tag = s.tag;
+				ns.step = 0;
+				ns.flags = 0;
+				ns.sid = se_sid(s.sid, s.step);
+				const bool spill = sslot >= se_limit;
+				const uint32_t pslot = wave_alloc(&C->spill_ctr, spill);
+				if (!spill) {
 					store_state(out, B - 1u - sslot, ns);
+				} else if (pslot < a.spill_cap) {
+					store_state(a.spill, pslot, ns);
 				} else {
-					// no room: keep the state before the event and retry it
+					// no room anywhere: keep the state before the event and
+					// retry it in a later launch
 					s = pre;
 					s.flags |= FL_PEND_INEL;
 					stop = true;
@@ -370,6 +394,9 @@ struct nbl_driver {
 	uint32_t capacity, slots, K, det_cap;
 	Bufs buf[2];
 	void* buf_mem[2];
+	Bufs spill;
+	void* spill_mem;
+	uint32_t spill_cap;
 	Ctrl* ctrl;
 	Ctrl* h_ctrl;
 	nbl_detected* det;
@@ -473,11 +500,11 @@ int read_ctrl(nbl_driver* d)
 	return NBL_OK;
 }
 
-uint32_t active_count(const Ctrl& c)
+uint32_t active_count(const Ctrl& c, uint32_t spill_cap)
 {
 	if (c.stepped)
-		return c.surv_ctr + std::min(c.se_ctr, c.se_limit);
-	return c.n_surv + c.n_se;
+		return c.surv_ctr + std::min(c.se_ctr, c.se_limit) + std::min(c.spill_ctr, spill_cap);
+	return c.n_surv + c.n_se + c.spill_n;
 }
 
 } // namespace
@@ -684,6 +711,7 @@ int nbl_driver_destroy(nbl_driver* d)
 	for (auto& pr : d->ev_free) { hipEventDestroy(pr.first); hipEventDestroy(pr.second); }
 	hipFree(d->buf_mem[0]);
 	hipFree(d->buf_mem[1]);
+	hipFree(d->spill_mem);
 	hipFree(d->ctrl);
 	hipHostFree(d->h_ctrl);
 	hipFree(d->det);
@@ -730,6 +758,8 @@ int nbl_driver_create(nbl_device* dev, uint32_t capacity, const nbl_geometry* ge
 	}
 	if ((rc = alloc_bufs(d->buf[0], d->buf_mem[0], d->slots))) return cleanup(rc);
 	if ((rc = alloc_bufs(d->buf[1], d->buf_mem[1], d->slots))) return cleanup(rc);
+	d->spill_cap = std::max<uint32_t>(1u << 20, std::min<uint64_t>(4ull * d->slots, 1ull << 28));
+	if ((rc = alloc_bufs(d->spill, d->spill_mem, d->spill_cap))) return cleanup(rc);
 	if ((rc = dev_alloc(&d->ctrl, 1))) return cleanup(rc);
 	if (hipHostMalloc((void**)&d->h_ctrl, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess)
 		return cleanup(fail(NBL_ERR_ALLOC, "hipHostMalloc failed"));
@@ -809,6 +839,8 @@ int nbl_driver_iterate(nbl_driver* d, uint32_t n_iter)
 	a.W = d->W;
 	a.buf[0] = d->buf[0];
 	a.buf[1] = d->buf[1];
+	a.spill = d->spill;
+	a.spill_cap = d->spill_cap;
 	a.ctrl = d->ctrl;
 	a.det = d->det;
 	a.det_cap = d->det_cap;
@@ -817,9 +849,9 @@ int nbl_driver_iterate(nbl_driver* d, uint32_t n_iter)
 	const uint32_t g_refill = blocks_for(d->capacity);
 	const uint32_t g_step = blocks_for(d->capacity);
 	for (uint32_t it = 0; it < n_iter; ++it) {
-		hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(64), 0, d->stream, d->ctrl, d->slots);
+		hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(64), 0, d->stream, d->ctrl, d->slots, d->spill_cap);
 		hipLaunchKernelGGL(k_refill, dim3(g_refill), dim3(256), 0, d->stream, d->ctrl, d->buf[0],
-			d->buf[1], d->q_part, d->q_tag, d->q_sid);
+			d->buf[1], d->spill, d->q_part, d->q_tag, d->q_sid);
 		std::pair<hipEvent_t, hipEvent_t> ev{};
 		if (d->timing) {
 			if (d->ev_free.empty()) {
@@ -852,7 +884,7 @@ int nbl_driver_counts(nbl_driver* d, uint32_t* running, uint32_t* detected, uint
 	if ((rc = read_ctrl(d)))
 		return rc;
 	const Ctrl& c = *d->h_ctrl;
-	if (running) *running = active_count(c);
+	if (running) *running = active_count(c, d->spill_cap);
 	if (detected) *detected = std::min(c.det_ctr, d->det_cap);
 	if (queued) *queued = c.queue_n - c.queue_next;
 	return NBL_OK;
@@ -901,6 +933,8 @@ int nbl_driver_run(nbl_driver* d, nbl_detected* out, uint64_t cap, uint64_t* n_d
 	uint64_t total = 0;
 	uint32_t frame = 4;
 	std::vector<nbl_detected> tmp;
+	uint64_t last_steps = ~0ull;
+	int stalled = 0;
 	for (;;) {
 		int rc = nbl_driver_iterate(d, frame);
 		if (rc) return rc;
@@ -925,6 +959,16 @@ int nbl_driver_run(nbl_driver* d, nbl_detected* out, uint64_t cap, uint64_t* n_d
 		}
 		if (running == 0 && queued == 0 && d->h_ctrl->det_ctr == 0)
 			break;
+		// Every particle waiting for a secondary slot with the buffer and the
+		// spill stack full would never finish: report it instead of spinning.
+		if (d->h_ctrl->steps == last_steps && det == 0) {
+			if (++stalled >= 4)
+				return fail(NBL_ERR_CAPACITY, "no progress: particle buffer and spill stack full of "
+					"particles waiting for secondary-electron slots; raise the capacity");
+		} else {
+			stalled = 0;
+		}
+		last_steps = d->h_ctrl->steps;
 		frame = std::min<uint32_t>(frame * 2, 32);
 	}
 	*n_detected = total;

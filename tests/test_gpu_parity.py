@@ -159,7 +159,7 @@ def test_cascades_match_oracle(dev, capacity, K):
     seed = 1
     g = A.Geometry(dev, tris)
     d = A.Driver(dev, capacity, g, [A.Material(dev, A.HostMaterial(raw))], seed=seed, steps_per_launch=K)
-    pri = W.pencil_beam(300, 1000.0)
+    pri = W.pencil_beam(300 if capacity > 64 else 100, 1000.0)
     d.push(pri)
     det_g = d.run(100000)
     ow = O.OracleWorld(tris, [raw])
