@@ -134,6 +134,11 @@ def main():
     ap.add_argument("--pixels", type=int, default=256)
     ap.add_argument("--per-pixel", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the run, copy 1 GiB device-to-device with a plain copy kernel "
+                         "(reference point for the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_step_traffic.json"),
+                    help="measured HBM bytes per k_step launch (from rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the tracking path shards by "
@@ -171,6 +176,20 @@ def main():
     drv.set_timing(False)
     n_steps = drv.steps() - steps0
     n_prim = args.steps * n
+    if args.calibrate_copy:
+        import ctypes
+        ms = ctypes.c_float()
+        A.check(A.lib().nbl_device_copy_probe(dev._h, 1 << 30, ctypes.byref(ms)))
+        print(f"copy probe: 1 GiB in {ms.value:.3f} ms", file=sys.stderr)
+    traffic = None
+    if os.path.exists(args.traffic_file):
+        with open(args.traffic_file) as f:
+            tr = json.load(f)
+        same = all(tr.get("config", {}).get(k) == v for k, v in
+                   (("pixels", args.pixels), ("per_pixel", args.per_pixel), ("capacity", args.capacity),
+                    ("steps_per_launch", args.steps_per_launch)))
+        if same:
+            traffic = tr["bytes_per_launch"]
     value = n_prim / t
     e = det["E"]
     bse = float(np.sum(e >= 50) / n_prim)
@@ -195,7 +214,7 @@ def main():
                    "steps_per_launch": args.steps_per_launch, "physics": "full_penn + kieft_elastic",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_step", "launches": launches,
                      "avg_launch_us": 1e3 * step_ms / max(launches, 1),
                      "alg_bytes_per_launch": b_alg / max(launches, 1)},

@@ -249,6 +249,11 @@ void* nbl_driver_stream(nbl_driver* d);
 int nbl_driver_set_timing(nbl_driver* d, int enable);
 int nbl_driver_timing(nbl_driver* d, double* step_ms_total, uint64_t* step_launches);
 
+/* Measurement aid (no reference counterpart): copy `bytes` between two
+ * scratch device buffers with a plain copy kernel (k_copy) and return its
+ * time; a known-size reference point for HBM traffic counters. */
+int nbl_device_copy_probe(nbl_device* dev, uint64_t bytes, float* ms);
+
 /* ------------------------------------------------------- test hooks ---- */
 /* Batched octree::propagate (geometry/octree.inl:32-192): ray i from pos[3i]
  * along dir[3i] (already normalised) over dist[i], ignoring triangle

@@ -142,6 +142,7 @@ _SIGS = {
                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
     "nbl_table_probe": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_uint32]),
+    "nbl_device_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, _p(C.c_float)]),
     "nbl_driver_step_states": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_uint32]),
 }

@@ -188,6 +188,8 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
 	const uint32_t n_surv = C->n_surv, n_in = n_surv + C->n_se;
 	const uint32_t B = a.slots, se_limit = C->se_limit, det_cap = a.det_cap;
 	const World& W = a.W;
+	__shared__ int s_path[PATH_LEVELS * 256];
+	int* path = s_path + threadIdx.x;
 	uint32_t my_steps = 0;
 	const uint32_t stride = gridDim.x * blockDim.x;
 	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_in; i += stride) {
@@ -211,7 +213,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
 					ev = EV_INELASTIC;
 					rng_seek(rng, 2);
 				} else {
-					oc = step_init(W, s, rng, ev, hit);
+					oc = step_init(W, s, rng, ev, hit, path, 256);
 					++my_steps;
 				}
 				pre = s;
@@ -284,18 +286,26 @@ __global__ void k_reset_det(Ctrl* c)
 		c->det_ctr = 0;
 }
 
+// Measurement aid: a plain grid-stride copy of 16-byte words.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n)
+{
+	for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+		dst[i] = src[i];
+}
+
 // Test hook: batched propagate.
 __global__ __launch_bounds__(256) void k_query(Geometry g, const float* pos, const float* dir,
 	const float* dist, const int* ignore_tri, const int* ignore_mat, int* out_tri,
 	float* out_dist, uint32_t n)
 {
+	__shared__ int s_path[PATH_LEVELS * 256];
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
 	const F3 p = { pos[3 * i], pos[3 * i + 1], pos[3 * i + 2] };
 	const F3 d = { dir[3 * i], dir[3 * i + 1], dir[3 * i + 2] };
 	float od;
-	out_tri[i] = propagate(g, p, d, dist[i], ignore_tri[i], ignore_mat[i], od);
+	out_tri[i] = propagate(g, p, d, dist[i], ignore_tri[i], ignore_mat[i], od, s_path + threadIdx.x, 256);
 	out_dist[i] = od;
 }
 
@@ -329,6 +339,7 @@ __global__ __launch_bounds__(256) void k_probe(Material m, int which, int mode, 
 __global__ __launch_bounds__(256) void k_step_states(World W, nbl_step_state* st, int* event,
 	int* outcome, nbl_step_state* se_out, int* se_made_out, uint32_t n)
 {
+	__shared__ int s_path[PATH_LEVELS * 256];
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
@@ -342,7 +353,7 @@ __global__ __launch_bounds__(256) void k_step_states(World W, nbl_step_state* st
 	Rng rng;
 	rng_start(rng, W.key0, W.key1, s.sid, s.step);
 	int ev = EV_NONE, hit = -1;
-	int oc = step_init(W, s, rng, ev, hit);
+	int oc = step_init(W, s, rng, ev, hit, s_path + threadIdx.x, 256);
 	Particle se;
 	bool made = false;
 	if (oc == ALIVE)
@@ -565,21 +576,27 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 	// inside it and every leaf entry at a triangle, so that the traversal
 	// kernel cannot read out of bounds.
 	{
+		// (node offset, depth); leaves may sit at depth 21 at most (the
+		// builder's limit, which the 64-bit location code and the per-level
+		// path cache of the walk rely on)
 		std::vector<uint8_t> seen((size_t)n_od, 0);
-		std::vector<int64_t> stack{ 0 };
+		std::vector<std::pair<int64_t, int>> stack{ { 0, 0 } };
 		seen[0] = 1;
 		while (!stack.empty()) {
-			const int64_t node = stack.back();
+			const int64_t node = stack.back().first;
+			const int depth = stack.back().second;
 			stack.pop_back();
 			if (node + 8 > n_od)
 				return fail(NBL_ERR_FORMAT, "octree node out of range at " + std::to_string(node));
+			if (depth + 1 > PATH_LEVELS - 1)
+				return fail(NBL_ERR_FORMAT, "octree deeper than 21 levels");
 			for (int o = 0; o < 8; ++o) {
 				const int64_t v = od[node + o];
 				if (v > 0) {
 					if (v >= n_od || seen[v])
 						return fail(NBL_ERR_FORMAT, "bad octree child offset at " + std::to_string(node + o));
 					seen[v] = 1;
-					stack.push_back(v);
+					stack.push_back({ v, depth + 1 });
 				} else {
 					int64_t p = -v;
 					if (p <= 0 || p >= n_od)
@@ -1083,6 +1100,35 @@ int nbl_driver_step_states(nbl_driver* d, nbl_step_state* states, int32_t* event
 	HIP_OK(hipMemcpyAsync(se_made, dev_made, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
 	HIP_OK(hipStreamSynchronize(s));
 	hipFree(dst); hipFree(dse); hipFree(dev_ev); hipFree(dev_oc); hipFree(dev_made);
+	return NBL_OK;
+}
+
+int nbl_device_copy_probe(nbl_device* dev, uint64_t bytes, float* ms)
+{
+	clear_error();
+	if (!dev || bytes < 16 || !ms)
+		return fail(NBL_ERR_ARG, "invalid copy probe arguments");
+	int rc = set_device(dev);
+	if (rc) return rc;
+	const size_t n = (size_t)(bytes / 16);
+	float4 *a = nullptr, *b = nullptr;
+	if ((rc = dev_alloc(&a, n)) || (rc = dev_alloc(&b, n))) {
+		hipFree(a);
+		return rc;
+	}
+	hipEvent_t e0, e1;
+	HIP_OK(hipEventCreate(&e0));
+	HIP_OK(hipEventCreate(&e1));
+	HIP_OK(hipMemsetAsync(a, 0, n * 16, dev->stream));
+	HIP_OK(hipEventRecord(e0, dev->stream));
+	hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, dev->stream, a, b, n);
+	HIP_OK(hipEventRecord(e1, dev->stream));
+	HIP_OK(hipEventSynchronize(e1));
+	HIP_OK(hipEventElapsedTime(ms, e0, e1));
+	hipEventDestroy(e0);
+	hipEventDestroy(e1);
+	hipFree(a);
+	hipFree(b);
 	return NBL_OK;
 }
 

@@ -321,10 +321,20 @@ __device__ __forceinline__ bool in_domain(const Geometry& g, F3 p)
 		&& (p.z > c.z - h.z) && (p.z < c.z + h.z);
 }
 
+// Levels of the octree table cached per thread: path[l * stride] holds the
+// node index at level l (1..21) of the current cell.
+constexpr int PATH_LEVELS = 22;
+
 // geometry/octree.inl:32-192: walk leaf cells by location code; returns the
 // hit triangle (or -1) and the distance.
+//
+// The reference re-descends from the root for every cell.  Here the node
+// indices of the current cell's ancestors are kept in LDS (`path`, one int
+// per level, lane-strided), so moving to a neighbour loads only the flipped
+// level and whatever lies below it.  The cell-centre arithmetic still runs
+// from the root in the reference's order, so every value is unchanged.
 __device__ inline int propagate(const Geometry& g, F3 start, F3 dir, float distance,
-	int ignore_tri, int ignore_mat, float& out_dist)
+	int ignore_tri, int ignore_mat, float& out_dist, int* __restrict__ path, int stride)
 {
 	const int* __restrict__ od = g.od;
 	const F3 hs = g.halfsize;
@@ -334,14 +344,21 @@ __device__ inline int propagate(const Geometry& g, F3 start, F3 dir, float dista
 	uint32_t cells = 0;
 	do {
 		float ax = g.center.x, ay = g.center.y, az = g.center.z, aw = 1.0f;
-		int index = 0;
-		for (int i = 60 - __clzll((long long)location); i >= 0; i -= 3) {
+		const int clz = __clzll((long long)location);
+		for (int i = 60 - clz; i >= 0; i -= 3) {
 			const unsigned oct = (unsigned)(location >> i) & 7u;
-			index = od[index + (int)oct];
 			ax += aw * hs.x * ((float)(oct & 1u) - 0.5f);
 			ay += aw * hs.y * ((float)((oct & 2u) >> 1) - 0.5f);
 			az += aw * hs.z * ((float)((oct & 4u) >> 2) - 0.5f);
 			aw *= 0.5f;
+		}
+		// levels 1..L-1 are unchanged ancestors; level L holds the new octant
+		int lev = (63 - clz) / 3;
+		int index = 0;
+		if (lev > 0) {
+			const int parent = (lev >= 2) ? path[(lev - 1) * stride] : 0;
+			index = od[parent + (int)((unsigned)location & 7u)];
+			path[lev * stride] = index;
 		}
 		while (index >= 0) {
 			unsigned oct = 0;
@@ -350,6 +367,7 @@ __device__ inline int propagate(const Geometry& g, F3 start, F3 dir, float dista
 			oct ^= (cur.z > az) ? 4u : 0u;
 			location = (location << 3) | oct;
 			index = od[index + (int)oct];
+			path[++lev * stride] = index;
 			ax += aw * hs.x * ((float)(oct & 1u) - 0.5f);
 			ay += aw * hs.y * ((float)((oct & 2u) >> 1) - 0.5f);
 			az += aw * hs.z * ((float)((oct & 4u) >> 2) - 0.5f);
@@ -710,7 +728,8 @@ struct World {
 // init (drivers/gpu/gpu_driver.inl:403-480): domain and energy checks, path
 // sampling, propagate, move; stores the normalised direction.  Returns ALIVE
 // or TERMINATED and the event; `hit` is the intersected triangle.
-__device__ inline int step_init(const World& W, State& s, Rng& rng, int& event, int& hit)
+__device__ inline int step_init(const World& W, State& s, Rng& rng, int& event, int& hit,
+	int* path, int stride)
 {
 	event = EV_NONE;
 	hit = -1;
@@ -733,7 +752,7 @@ __device__ inline int step_init(const World& W, State& s, Rng& rng, int& event, 
 	}
 	const F3 nd = normalised(s.p.dir);
 	float hd;
-	hit = propagate(W.g, s.p.pos, nd, distance, s.last_tri, s.mat, hd);
+	hit = propagate(W.g, s.p.pos, nd, distance, s.last_tri, s.mat, hd, path, stride);
 	s.p.dir = nd;
 	if (hit < 0) {
 		s.p.pos = v_add(s.p.pos, v_mul(nd, distance));
