@@ -110,7 +110,7 @@ def cpu_baseline(mesh, pri_full, mats, target_s=20.0, threads=None):
     sys.path.insert(0, ROOT)
     from oracle import pyoracle as O
     tris = A.mesh_to_triangles(mesh)
-    ow = O.OracleWorld(tris, mats, kind="dr")
+    ow = O.OracleWorld(tris, mats, kind="pf")
     side = 8
     while True:
         sample = crop(side)

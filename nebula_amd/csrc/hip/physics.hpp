@@ -4,7 +4,7 @@
 // function cited beside it (paths relative to the reference's source/), and
 // the library is compiled with -ffp-contract=off, so each step is
 // bit-reproducible against a CPU evaluation of the same expressions.
-// Transcendentals are evaluated in double and rounded to float.
+// Transcendentals are the float routines of this file (see "math").
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,13 +37,172 @@ constexpr uint32_t FL_PEND_INEL = 1u, FL_PEND_DET = 2u;
 struct F3 { float x, y, z; };
 
 // ------------------------------------------------------------------ math --
-__device__ __forceinline__ float m_log(float x) { return (float)log((double)x); }
-__device__ __forceinline__ float m_exp(float x) { return (float)exp((double)x); }
-__device__ __forceinline__ float m_expm1(float x) { return (float)expm1((double)x); }
-__device__ __forceinline__ float m_cbrt(float x) { return (float)cbrt((double)x); }
-__device__ __forceinline__ float m_sin(float x) { return (float)sin((double)x); }
-__device__ __forceinline__ float m_cos(float x) { return (float)cos((double)x); }
-__device__ __forceinline__ float m_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+// Float transcendentals built from IEEE float +, -, *, / and integer bit
+// operations only, so that (compiled without FMA contraction) they give the
+// same bits on the GPU and in a CPU evaluation.  Methods: exponent/mantissa
+// split with log(1+f) = f - f^2/2 + s(f^2/2 + R(s^2)), s = f/(2+f), R the
+// atanh series; Cody-Waite reduction by k ln2 or k pi/2 with split constants
+// and Taylor polynomials; arctangent by reduction to |u| <= tan(pi/8) and a
+// least-squares polynomial; cube root by a bit-level first guess and four
+// Newton steps.  All within 2.5 ulp of the exact functions.
+__device__ __forceinline__ float pf_nan() { return __uint_as_float(0x7fc00000u); }
+__device__ __forceinline__ float pf_pow2(int k) { return __uint_as_float((uint32_t)(k + 127) << 23); }
+constexpr float PF_P1 = 0x1.922p+0f, PF_P2 = -0x1.2aep-18f, PF_P3 = -0x1.de973ep-31f;
+constexpr float PF_LN2_HI = 0x1.62e4p-1f, PF_LN2_LO = 0x1.7f7d1cp-20f, PF_INV_LN2 = 0x1.715476p+0f;
+constexpr float PF_PI_HI = 0x1.921fb6p+1f, PF_PI_LO = -0x1.777a5cp-24f;
+constexpr float PF_PI_2_HI = 0x1.921fb6p+0f, PF_PI_2_LO = -0x1.777a5cp-25f;
+constexpr float PF_PI_4_HI = 0x1.921fb6p-1f, PF_PI_4_LO = -0x1.777a5cp-26f;
+constexpr float PF_TWO_OVER_PI = 0x1.45f306p-1f, PF_TAN_PI_8 = 0x1.a8279ap-2f;
+
+__device__ inline float pf_log(float x)
+{
+	if (x != x) return x;
+	if (x < 0.0f) return pf_nan();
+	if (x == 0.0f) return -__builtin_inff();
+	uint32_t i = __float_as_uint(x);
+	if (i >= 0x7f800000u) return x;
+	int e = 0;
+	if (i < 0x00800000u) { x = x * 0x1p25f; i = __float_as_uint(x); e = -25; }
+	e += (int)(i >> 23) - 127;
+	const uint32_t m = i & 0x007fffffu;
+	if (m > 0x3504f3u) { i = m | 0x3f000000u; e += 1; }
+	else i = m | 0x3f800000u;
+	const float f = __uint_as_float(i) - 1.0f;
+	const float s = f / (2.0f + f);
+	const float z = s * s;
+	const float R = z * (0x1.555556p-1f + z * (0x1.99999ap-2f + z * (0x1.24924ap-2f + z * 0x1.c71c72p-3f)));
+	const float hfsq = 0.5f * f * f;
+	const float l1p = f - (hfsq - s * (hfsq + R));
+	const float fe = (float)e;
+	return fe * PF_LN2_HI + (l1p + fe * PF_LN2_LO);
+}
+
+__device__ __forceinline__ float pf_exp_p(float r)
+{
+	return 0.5f + r * (0x1.555556p-3f + r * (0x1.555556p-5f + r * (0x1.111112p-7f
+		+ r * (0x1.6c16c2p-10f + r * 0x1.a01a02p-13f))));
+}
+
+__device__ inline float pf_exp(float x)
+{
+	if (x != x) return x;
+	if (x > 0x1.62e42ep+6f) return __builtin_inff();
+	if (x < -0x1.9fe368p+6f) return 0.0f;
+	const float kf = rintf(x * PF_INV_LN2);
+	const int k = (int)kf;
+	const float r = (x - kf * PF_LN2_HI) - kf * PF_LN2_LO;
+	const float e = 1.0f + (r + (r * r) * pf_exp_p(r));
+	if (k > 127) return (e * 0x1p127f) * pf_pow2(k - 127);
+	if (k < -126) return (e * 0x1p-126f) * pf_pow2(k + 126);
+	return e * pf_pow2(k);
+}
+
+__device__ inline float pf_expm1(float x)
+{
+	if (x != x) return x;
+	if (x > 0x1.62e42ep+6f) return __builtin_inff();
+	if (x < -0x1.154246p+4f) return -1.0f;
+	if (fabsf(x) < 0x1.62e430p-2f) {
+		if (fabsf(x) < 0x1p-28f) return x;
+		return x + (x * x) * pf_exp_p(x);
+	}
+	const float kf = rintf(x * PF_INV_LN2);
+	const int k = (int)kf;
+	const float r = (x - kf * PF_LN2_HI) - kf * PF_LN2_LO;
+	const float em = r + (r * r) * pf_exp_p(r);
+	if (k > 127) return ((em + 1.0f) * 0x1p127f) * 2.0f - 1.0f;
+	const float t = pf_pow2(k);
+	return em * t + (t - 1.0f);
+}
+
+__device__ inline float pf_cbrt(float x)
+{
+	if (x != x || x == 0.0f || fabsf(x) == __builtin_inff()) return x;
+	float ax = fabsf(x);
+	float post = 1.0f;
+	if (ax < 0x1p-126f) { ax = ax * 0x1p24f; post = 0x1p-8f; }
+	float y = __uint_as_float(__float_as_uint(ax) / 3u + 0x2a555555u);
+#pragma unroll
+	for (int it = 0; it < 4; ++it)
+		y = (2.0f * y + ax / (y * y)) / 3.0f;
+	y = y * post;
+	return copysignf(y, x);
+}
+
+__device__ __forceinline__ float pf_sin_r(float r)
+{
+	const float z = r * r;
+	return r + (r * z) * (-0x1.555556p-3f + z * (0x1.111112p-7f + z * (-0x1.a01a02p-13f
+		+ z * (0x1.71de3ap-19f + z * -0x1.ae6456p-26f))));
+}
+__device__ __forceinline__ float pf_cos_r(float r)
+{
+	const float z = r * r;
+	return 1.0f - (0.5f * z - (z * z) * (0x1.555556p-5f + z * (-0x1.6c16c2p-10f
+		+ z * (0x1.a01a02p-16f + z * -0x1.27e4fcp-22f))));
+}
+__device__ __forceinline__ float pf_reduce(float x, int& q)
+{
+	const float kf = rintf(x * PF_TWO_OVER_PI);
+	q = ((int)kf) & 3;
+	return ((x - kf * PF_P1) - kf * PF_P2) - kf * PF_P3;
+}
+// sin and cos of the same argument (the reference calls sincos on the CPU
+// as separate sin and cos, which give the same values as these)
+__device__ inline void pf_sincos(float x, float& s, float& c)
+{
+	if (x != x || fabsf(x) == __builtin_inff()) { s = pf_nan(); c = pf_nan(); return; }
+	int q;
+	const float r = pf_reduce(x, q);
+	const float sr = pf_sin_r(r), cr = pf_cos_r(r);
+	switch (q) {
+	case 0: s = sr; c = cr; break;
+	case 1: s = cr; c = -sr; break;
+	case 2: s = -sr; c = -cr; break;
+	default: s = -cr; c = sr; break;
+	}
+}
+
+__device__ __forceinline__ float pf_atan_r(float u)
+{
+	const float z = u * u;
+	return u + (u * z) * (-0x1.555556p-2f + z * (0x1.999982p-3f + z * (-0x1.248b8p-3f
+		+ z * (0x1.c5adfep-4f + z * (-0x1.62ae6ep-4f + z * 0x1.a49afap-5f)))));
+}
+__device__ inline float pf_atan2(float y, float x)
+{
+	if (x != x || y != y) return x + y;
+	const float ax = fabsf(x), ay = fabsf(y);
+	const bool xneg = (__float_as_uint(x) >> 31) != 0;
+	if (ay == 0.0f)
+		return xneg ? copysignf(PF_PI_HI, y) : y;
+	if (ax == 0.0f)
+		return copysignf(PF_PI_2_HI, y);
+	const float inf = __builtin_inff();
+	if (ax == inf || ay == inf) {
+		float a;
+		if (ax == inf && ay == inf) a = xneg ? 3.0f * PF_PI_4_HI : PF_PI_4_HI;
+		else if (ax == inf) a = xneg ? PF_PI_HI : 0.0f;
+		else a = PF_PI_2_HI;
+		return copysignf(a, y);
+	}
+	const bool swap = ay > ax;
+	const float t = swap ? ax / ay : ay / ax;
+	float a;
+	if (t > PF_TAN_PI_8)
+		a = PF_PI_4_HI + (pf_atan_r((t - 1.0f) / (t + 1.0f)) + PF_PI_4_LO);
+	else
+		a = pf_atan_r(t);
+	if (swap) a = (PF_PI_2_HI - a) + PF_PI_2_LO;
+	if (xneg) a = (PF_PI_HI - a) + PF_PI_LO;
+	return copysignf(a, y);
+}
+
+__device__ __forceinline__ float m_log(float x) { return pf_log(x); }
+__device__ __forceinline__ float m_exp(float x) { return pf_exp(x); }
+__device__ __forceinline__ float m_expm1(float x) { return pf_expm1(x); }
+__device__ __forceinline__ float m_cbrt(float x) { return pf_cbrt(x); }
+__device__ __forceinline__ float m_atan2(float y, float x) { return pf_atan2(y, x); }
 
 // common/scalar_math.h:16-25; float->int conversions follow the x86 CPU
 // build for out-of-range and NaN inputs (INT_MIN).
@@ -80,16 +239,19 @@ __device__ __forceinline__ F3 normalised(F3 a)
 __device__ inline F3 make_normal_vec(F3 dir, float phi)
 {
 	const float az = m_atan2(dir.y, dir.x);
-	const float sin_az = m_sin(az), cos_az = m_cos(az);
+	float sin_az, cos_az;
+	pf_sincos(az, sin_az, cos_az);
 	const F3 unit_v = { dir.z * cos_az, dir.z * sin_az, -sqrtf(dir.x * dir.x + dir.y * dir.y) };
 	const F3 unit_u = cross(unit_v, dir);
-	const float sin_phi = m_sin(phi), cos_phi = m_cos(phi);
+	float sin_phi, cos_phi;
+	pf_sincos(phi, sin_phi, cos_phi);
 	return v_add(v_mul(unit_u, cos_phi), v_mul(unit_v, sin_phi));
 }
 // vector_math.h:194-206
 __device__ inline F3 make_unit_vec(float cos_theta, float phi)
 {
-	const float sin_phi = m_sin(phi), cos_phi = m_cos(phi);
+	float sin_phi, cos_phi;
+	pf_sincos(phi, sin_phi, cos_phi);
 	cos_theta = clampr(cos_theta, -1.0f, 1.0f);
 	const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
 	return { sin_theta * cos_phi, sin_theta * sin_phi, cos_theta };

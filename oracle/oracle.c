@@ -31,16 +31,215 @@ static const float PI_F = 3.1415926535897932f;
 
 /* ------------------------------------------------------------- math mode -- */
 /* Transcendentals of the tracking loop.  Material preprocessing (create)
- * always uses the C library, as the host code of the reference does. */
-#ifdef ORC_MATH_DR
-static inline float m_log(float x) { return (float)log((double)x); }
-static inline float m_exp(float x) { return (float)exp((double)x); }
-static inline float m_expm1(float x) { return (float)expm1((double)x); }
-static inline float m_cbrt(float x) { return (float)cbrt((double)x); }
-static inline float m_sin(float x) { return (float)sin((double)x); }
-static inline float m_cos(float x) { return (float)cos((double)x); }
-static inline float m_atan2(float y, float x) { return (float)atan2((double)y, (double)x); }
-int orc_math_mode(void) { return 1; }
+ * always uses the C library, as the host code of the reference does.
+ *
+ * ORC_MATH_PF: the float transcendentals of the HIP kernels, restated.  They
+ * use only IEEE float +, -, *, / and integer bit operations, so with
+ * -ffp-contract=off the CPU and the GPU produce the same bits.  Methods:
+ * exponent/mantissa split with the log(1+f) = f - f^2/2 + s(f^2/2 + R(s^2))
+ * identity (s = f/(2+f), R the atanh series); Cody-Waite reduction by
+ * k*ln2 or k*pi/2 with split constants and Taylor polynomials; arctangent
+ * by reduction to |u| <= tan(pi/8) and a least-squares polynomial; cube root
+ * by a bit-level first guess and four Newton steps.  Accuracy against the
+ * exact functions is checked by tests/test_math.py. */
+#ifdef ORC_MATH_PF
+#define PF_P1 0x1.922p+0f
+#define PF_P2 -0x1.2aep-18f
+#define PF_P3 -0x1.de973ep-31f
+#define PF_LN2_HI 0x1.62e4p-1f
+#define PF_LN2_LO 0x1.7f7d1cp-20f
+#define PF_INV_LN2 0x1.715476p+0f
+#define PF_PI_HI 0x1.921fb6p+1f
+#define PF_PI_LO -0x1.777a5cp-24f
+#define PF_PI_2_HI 0x1.921fb6p+0f
+#define PF_PI_2_LO -0x1.777a5cp-25f
+#define PF_PI_4_HI 0x1.921fb6p-1f
+#define PF_PI_4_LO -0x1.777a5cp-26f
+#define PF_TWO_OVER_PI 0x1.45f306p-1f
+#define PF_TAN_PI_8 0x1.a8279ap-2f
+
+static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static inline float pf_nan(void) { return u2f(0x7fc00000u); }
+/* 2^k for -126 <= k <= 127 */
+static inline float pf_pow2(int k) { return u2f((uint32_t)(k + 127) << 23); }
+
+static float pf_log(float x)
+{
+	if (x != x) return x;
+	if (x < 0.0f) return pf_nan();
+	if (x == 0.0f) return -INFINITY;
+	uint32_t i = f2u(x);
+	if (i >= 0x7f800000u) return x;
+	int e = 0;
+	if (i < 0x00800000u) { x = x * 0x1p25f; i = f2u(x); e = -25; }
+	e += (int)(i >> 23) - 127;
+	const uint32_t m = i & 0x007fffffu;
+	if (m > 0x3504f3u) { i = m | 0x3f000000u; e += 1; }   /* mantissa above sqrt(2): halve */
+	else i = m | 0x3f800000u;
+	const float f = u2f(i) - 1.0f;
+	const float s = f / (2.0f + f);
+	const float z = s * s;
+	const float R = z * (0x1.555556p-1f + z * (0x1.99999ap-2f + z * (0x1.24924ap-2f + z * 0x1.c71c72p-3f)));
+	const float hfsq = 0.5f * f * f;
+	const float l1p = f - (hfsq - s * (hfsq + R));
+	const float fe = (float)e;
+	return fe * PF_LN2_HI + (l1p + fe * PF_LN2_LO);
+}
+
+/* (exp(r) - 1 - r) / r^2, |r| <= ln2/2 */
+static inline float pf_exp_p(float r)
+{
+	return 0.5f + r * (0x1.555556p-3f + r * (0x1.555556p-5f + r * (0x1.111112p-7f
+		+ r * (0x1.6c16c2p-10f + r * 0x1.a01a02p-13f))));
+}
+
+static float pf_exp(float x)
+{
+	if (x != x) return x;
+	if (x > 0x1.62e42ep+6f) return INFINITY;
+	if (x < -0x1.9fe368p+6f) return 0.0f;
+	const float kf = rintf(x * PF_INV_LN2);
+	const int k = (int)kf;
+	const float r = (x - kf * PF_LN2_HI) - kf * PF_LN2_LO;
+	const float e = 1.0f + (r + (r * r) * pf_exp_p(r));
+	if (k > 127) return (e * 0x1p127f) * pf_pow2(k - 127);
+	if (k < -126) return (e * 0x1p-126f) * pf_pow2(k + 126);
+	return e * pf_pow2(k);
+}
+
+static float pf_expm1(float x)
+{
+	if (x != x) return x;
+	if (x > 0x1.62e42ep+6f) return INFINITY;
+	if (x < -0x1.154246p+4f) return -1.0f;     /* below -25 ln2 */
+	if (fabsf(x) < 0x1.62e430p-2f) {           /* |x| < ln2/2 */
+		if (fabsf(x) < 0x1p-28f) return x;
+		return x + (x * x) * pf_exp_p(x);
+	}
+	const float kf = rintf(x * PF_INV_LN2);
+	const int k = (int)kf;
+	const float r = (x - kf * PF_LN2_HI) - kf * PF_LN2_LO;
+	const float em = r + (r * r) * pf_exp_p(r);
+	if (k > 127) return ((em + 1.0f) * 0x1p127f) * 2.0f - 1.0f;
+	const float t = pf_pow2(k);
+	return em * t + (t - 1.0f);
+}
+
+static float pf_cbrt(float x)
+{
+	if (x != x || x == 0.0f || fabsf(x) == INFINITY) return x;
+	float ax = fabsf(x);
+	float post = 1.0f;
+	if (ax < 0x1p-126f) { ax = ax * 0x1p24f; post = 0x1p-8f; }
+	float y = u2f(f2u(ax) / 3u + 0x2a555555u);
+	for (int it = 0; it < 4; ++it)
+		y = (2.0f * y + ax / (y * y)) / 3.0f;
+	y = y * post;
+	return copysignf(y, x);
+}
+
+/* sin and cos of the reduced argument, |r| <= pi/4 */
+static inline float pf_sin_r(float r)
+{
+	const float z = r * r;
+	return r + (r * z) * (-0x1.555556p-3f + z * (0x1.111112p-7f + z * (-0x1.a01a02p-13f
+		+ z * (0x1.71de3ap-19f + z * -0x1.ae6456p-26f))));
+}
+static inline float pf_cos_r(float r)
+{
+	const float z = r * r;
+	return 1.0f - (0.5f * z - (z * z) * (0x1.555556p-5f + z * (-0x1.6c16c2p-10f
+		+ z * (0x1.a01a02p-16f + z * -0x1.27e4fcp-22f))));
+}
+/* x = k pi/2 + r */
+static inline float pf_reduce(float x, int* q)
+{
+	const float kf = rintf(x * PF_TWO_OVER_PI);
+	*q = ((int)kf) & 3;
+	return ((x - kf * PF_P1) - kf * PF_P2) - kf * PF_P3;
+}
+static float pf_sin(float x)
+{
+	if (x != x || fabsf(x) == INFINITY) return pf_nan();
+	int q;
+	const float r = pf_reduce(x, &q);
+	switch (q) {
+	case 0: return pf_sin_r(r);
+	case 1: return pf_cos_r(r);
+	case 2: return -pf_sin_r(r);
+	default: return -pf_cos_r(r);
+	}
+}
+static float pf_cos(float x)
+{
+	if (x != x || fabsf(x) == INFINITY) return pf_nan();
+	int q;
+	const float r = pf_reduce(x, &q);
+	switch (q) {
+	case 0: return pf_cos_r(r);
+	case 1: return -pf_sin_r(r);
+	case 2: return -pf_cos_r(r);
+	default: return pf_sin_r(r);
+	}
+}
+
+/* atan(u), |u| <= tan(pi/8) */
+static inline float pf_atan_r(float u)
+{
+	const float z = u * u;
+	return u + (u * z) * (-0x1.555556p-2f + z * (0x1.999982p-3f + z * (-0x1.248b8p-3f
+		+ z * (0x1.c5adfep-4f + z * (-0x1.62ae6ep-4f + z * 0x1.a49afap-5f)))));
+}
+static float pf_atan2(float y, float x)
+{
+	if (x != x || y != y) return x + y;
+	const float ax = fabsf(x), ay = fabsf(y);
+	const int xneg = signbit(x) != 0;
+	if (ay == 0.0f)
+		return xneg ? copysignf(PF_PI_HI, y) : y;
+	if (ax == 0.0f)
+		return copysignf(PF_PI_2_HI, y);
+	if (ax == INFINITY || ay == INFINITY) {
+		float a;
+		if (ax == INFINITY && ay == INFINITY) a = xneg ? 3.0f * PF_PI_4_HI : PF_PI_4_HI;
+		else if (ax == INFINITY) a = xneg ? PF_PI_HI : 0.0f;
+		else a = PF_PI_2_HI;
+		return copysignf(a, y);
+	}
+	const int swap = ay > ax;
+	const float t = swap ? ax / ay : ay / ax;
+	float a;
+	if (t > PF_TAN_PI_8)
+		a = PF_PI_4_HI + (pf_atan_r((t - 1.0f) / (t + 1.0f)) + PF_PI_4_LO);
+	else
+		a = pf_atan_r(t);
+	if (swap) a = (PF_PI_2_HI - a) + PF_PI_2_LO;
+	if (xneg) a = (PF_PI_HI - a) + PF_PI_LO;
+	return copysignf(a, y);
+}
+
+static inline float m_log(float x) { return pf_log(x); }
+static inline float m_exp(float x) { return pf_exp(x); }
+static inline float m_expm1(float x) { return pf_expm1(x); }
+static inline float m_cbrt(float x) { return pf_cbrt(x); }
+static inline float m_sin(float x) { return pf_sin(x); }
+static inline float m_cos(float x) { return pf_cos(x); }
+static inline float m_atan2(float y, float x) { return pf_atan2(y, x); }
+int orc_math_mode(void) { return 2; }
+
+float orc_pf_eval(int fn, float a, float b)
+{
+	switch (fn) {
+	case 0: return pf_log(a);
+	case 1: return pf_exp(a);
+	case 2: return pf_expm1(a);
+	case 3: return pf_cbrt(a);
+	case 4: return pf_sin(a);
+	case 5: return pf_cos(a);
+	default: return pf_atan2(a, b);
+	}
+}
 #else
 static inline float m_log(float x) { return logf(x); }
 static inline float m_exp(float x) { return expf(x); }
@@ -50,6 +249,19 @@ static inline float m_sin(float x) { return sinf(x); }
 static inline float m_cos(float x) { return cosf(x); }
 static inline float m_atan2(float y, float x) { return atan2f(y, x); }
 int orc_math_mode(void) { return 0; }
+
+float orc_pf_eval(int fn, float a, float b)
+{
+	switch (fn) {
+	case 0: return logf(a);
+	case 1: return expf(a);
+	case 2: return expm1f(a);
+	case 3: return cbrtf(a);
+	case 4: return sinf(a);
+	case 5: return cosf(a);
+	default: return atan2f(a, b);
+	}
+}
 #endif
 
 /* common/scalar_math.h:16-25 */
@@ -1589,4 +1801,10 @@ int64_t orc_run_philox(const orc_world* w, const orc_particle* primaries,
 	if (n_steps) *n_steps = atomic_load(&job.steps);
 	if (atomic_load(&job.overflow)) return -1;
 	return (int64_t)atomic_load(&job.nout);
+}
+
+void orc_pf_eval_batch(int fn, const float* a, const float* b, float* out, size_t n)
+{
+	for (size_t i = 0; i < n; ++i)
+		out[i] = orc_pf_eval(fn, a[i], b ? b[i] : 0.0f);
 }

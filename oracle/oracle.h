@@ -14,11 +14,9 @@
  *   liboracle_ref.so  transcendentals are the C library's float functions, as
  *                     std::log(float) etc. in the reference CPU build.  Pinned
  *                     bit-exactly against the reference compiled from source.
- *   liboracle.so      transcendentals are evaluated in double and rounded to
- *                     float ("DR math"), the same policy as the HIP kernels, so
- *                     HIP and oracle trajectories agree bit-for-bit except when
- *                     two libraries' double results straddle a float rounding
- *                     boundary (probability ~2^-29 per call).
+ *   liboracle.so      transcendentals are the same IEEE-float-only routines as
+ *                     the HIP kernels' (range reduction + polynomials), so HIP
+ *                     and oracle trajectories agree bit for bit.
  *
  * Pinning summary (what checks each part; tests/ hold the checks):
  *   tables, triangle test, octree build + propagate, material preprocessing,
@@ -146,7 +144,11 @@ enum { ORC_EV_NONE = 0, ORC_EV_INELASTIC = 1, ORC_EV_ELASTIC = 2,
 
 /* ------------------------------------------------------------- utilities -- */
 
-int orc_math_mode(void);  /* 0 = C library float math, 1 = DR math */
+int orc_math_mode(void);  /* 0 = C library float math, 2 = portable float math */
+/* Evaluate one transcendental of this build: fn 0 log, 1 exp, 2 expm1,
+ * 3 cbrt, 4 sin, 5 cos, 6 atan2(a, b). */
+float orc_pf_eval(int fn, float a, float b);
+void orc_pf_eval_batch(int fn, const float* a, const float* b, float* out, size_t n);
 
 void orc_mt_seed(orc_rng* r, uint32_t seed);
 uint32_t orc_mt_next(orc_rng* r);

@@ -3,7 +3,8 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, never by nebula_amd.  Two builds are loadable:
 "ref" (C library float math, bit-exact against the reference CPU build) and
-"dr" (double-rounded math, the HIP kernels' policy).
+"pf" (portable float math, the HIP kernels' policy: transcendentals built
+from IEEE float operations and bit operations only).
 """
 from __future__ import annotations
 
@@ -106,10 +107,10 @@ TRI_DTYPE = np.dtype([("mat_in", "<i4"), ("mat_out", "<i4"), ("r0", "<f4", 3), (
 _libs = {}
 
 
-def load(kind: str = "dr"):
-    """kind 'dr' -> liboracle.so, 'ref' -> liboracle_ref.so."""
+def load(kind: str = "pf"):
+    """kind 'pf' -> liboracle.so, 'ref' -> liboracle_ref.so."""
     if kind not in _libs:
-        path = os.path.join(BUILD, "liboracle.so" if kind == "dr" else "liboracle_ref.so")
+        path = os.path.join(BUILD, "liboracle.so" if kind == "pf" else "liboracle_ref.so")
         if not os.path.exists(path):
             raise ImportError(f"{path} not built: run `make -C oracle`")
         L = C.CDLL(path)
@@ -190,7 +191,7 @@ def parse_unit(text):
 class OracleMaterial:
     """orc_material_create of a workloads.RawMaterial."""
 
-    def __init__(self, raw, kind="dr"):
+    def __init__(self, raw, kind="pf"):
         self.L = load(kind)
         self.keep = []
         r = RawMaterial()
@@ -258,7 +259,7 @@ class OracleMaterial:
             pass
 
 
-def build_octree(tris: np.ndarray, kind="dr"):
+def build_octree(tris: np.ndarray, kind="pf"):
     L = load(kind)
     tris = np.ascontiguousarray(tris, dtype=TRI_DTYPE)
     od = C.POINTER(C.c_int32)()
@@ -292,7 +293,7 @@ def opts_from_flags(flags: int) -> dict:
 class OracleWorld:
     """Geometry (octree built by the oracle) + materials + options."""
 
-    def __init__(self, tris: np.ndarray, raw_materials, kind="dr", energy_threshold=0.0, opts=None,
+    def __init__(self, tris: np.ndarray, raw_materials, kind="pf", energy_threshold=0.0, opts=None,
                  octree=None):
         self.L = load(kind)
         self.kind = kind

@@ -83,7 +83,7 @@ def test_material_errors(tmp_path):
 
 def test_philox_known_answers():
     """Random123 known-answer vectors for Philox4x32-10."""
-    L = O.load("dr")
+    L = O.load("pf")
     out = (C.c_uint32 * 4)()
     kat = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
            ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
@@ -106,7 +106,7 @@ def test_mt19937_known_answer():
 
 def test_uniform_draws_in_half_open_unit_interval():
     """Philox uniforms are ((w >> 8) + 1) / 2^24: never 0, exactly 1 possible."""
-    L = O.load("dr")
+    L = O.load("pf")
     r = O.Rng()
     L.orc_rng_init_philox(C.byref(r), 99)
     L.orc_rng_set_stream(C.byref(r), 12345, 7)

@@ -53,7 +53,7 @@ def test_octree_matches_reference(name):
 
 
 @pytest.mark.parametrize("name", ["slab", "grating_small"])
-@pytest.mark.parametrize("kind", ["ref", "dr"])
+@pytest.mark.parametrize("kind", ["ref", "pf"])
 def test_propagate_matches_reference(name, kind):
     tris = A.mesh_to_triangles(MESHES[name]())
     ow = O.OracleWorld(tris, [W.make_material(W.SI)], kind=kind)

@@ -117,7 +117,7 @@ def test_yields_within_3sigma_of_reference(case):
     pri = indexed(beam())
     n = len(pri)
     E_ref, own_ref = run_reference(mesh, raws, pri, seed=11, model=model)
-    ow = O.OracleWorld(A.mesh_to_triangles(mesh), raws, kind="dr")
+    ow = O.OracleWorld(A.mesh_to_triangles(mesh), raws, kind="pf")
     det, _ = ow.run_philox(pri, 11, 50 * n, threads=8)
     b1, s1 = per_primary(E_ref, own_ref, n)
     b2, s2 = per_primary(det["E"], det["tag"].astype(np.int64), n)
