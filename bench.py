@@ -130,7 +130,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--capacity", type=int, default=1 << 21)
-    ap.add_argument("--steps-per-launch", type=int, default=4)
+    ap.add_argument("--steps-per-launch", type=int, default=8)
     ap.add_argument("--pixels", type=int, default=256)
     ap.add_argument("--per-pixel", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -138,7 +138,7 @@ def main():
                     help="after the run, copy 1 GiB device-to-device with a plain copy kernel "
                          "(reference point for the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_step_traffic.json"),
-                    help="measured HBM bytes per k_step launch (from rocprofv3 --pmc runs)")
+                    help="measured HBM bytes per k_step + k_inel launch pair (from rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the tracking path shards by "
@@ -215,7 +215,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_step", "launches": launches,
+                     "kernel": "k_step+k_inel", "launches": launches,
                      "avg_launch_us": 1e3 * step_ms / max(launches, 1),
                      "alg_bytes_per_launch": b_alg / max(launches, 1)},
         "cpu_baseline": cpu,

@@ -47,7 +47,7 @@ struct Ctrl {
 	uint32_t spill_n;       // secondaries parked on the spill stack
 	uint32_t spill_ctr;     // spill stack top during a k_step (may overshoot)
 	uint32_t spill_take, spill_src;  // refill from the spill stack
-	uint32_t pad0;
+	uint32_t inel_ctr;      // particles queued for their inelastic event
 	uint64_t refill_src;
 	uint64_t queue_n, queue_next;
 	uint64_t steps;         // particle-steps (init calls on active particles)
@@ -57,6 +57,7 @@ struct StepArgs {
 	World W;
 	Bufs buf[2];
 	Bufs spill;
+	Bufs inel;
 	Ctrl* ctrl;
 	nbl_detected* det;
 	uint32_t det_cap;
@@ -142,6 +143,7 @@ __global__ void k_ctrl(Ctrl* c, uint32_t slots, uint32_t spill_cap)
 	c->se_limit = slots - (c->n_surv + c->n_se);
 	c->surv_ctr = 0;
 	c->se_ctr = 0;
+	c->inel_ctr = 0;
 	c->spill_ctr = c->spill_n;
 	c->stepped = 1;
 }
@@ -179,6 +181,49 @@ __global__ __launch_bounds__(256) void k_refill(Ctrl* c, Bufs b0, Bufs b1, Bufs 
 	}
 }
 
+// Particle-step count: wave reduction, one atomic per wave.
+__device__ __forceinline__ void count_steps(Ctrl* C, uint32_t my_steps)
+{
+	uint32_t v = my_steps;
+	for (int off = 32; off > 0; off >>= 1)
+		v += (uint32_t)__shfl_xor((int)v, off);
+	if (__lane_id() == 0 && v)
+		atomicAdd((unsigned long long*)&C->steps, (unsigned long long)v);
+}
+
+// Place a new secondary: the output buffer's SE region, else the spill
+// stack; returns false when both are full.
+__device__ __forceinline__ bool place_secondary(const StepArgs& a, Ctrl* C, const Bufs& out,
+	uint32_t se_limit, bool made, const Particle& se, const State& parent)
+{
+	const uint32_t sslot = wave_alloc(&C->se_ctr, made);
+	bool ok = true;
+	if (made) {
+		State ns;
+		ns.p = se;
+		ns.mat = parent.mat;
+		ns.last_tri = -1;
+		ns.tag = parent.tag;
+		ns.step = 0;
+		ns.flags = 0;
+		ns.sid = se_sid(parent.sid, parent.step);
+		const bool spill = sslot >= se_limit;
+		const uint32_t pslot = wave_alloc(&C->spill_ctr, spill);
+		if (!spill)
+			store_state(out, a.slots - 1u - sslot, ns);
+		else if (pslot < a.spill_cap)
+			store_state(a.spill, pslot, ns);
+		else
+			ok = false;
+	}
+	return ok;
+}
+
+// Up to K steps per particle: init (domain, threshold, path sampling, octree
+// walk, move) and the boundary or elastic event.  A particle whose event is
+// inelastic stops and joins the inelastic queue, so that the long inelastic
+// code runs in k_inel on waves where every lane needs it (about 7% of the
+// steps of a 300 eV scan would otherwise put it into nearly every wave).
 __global__ __launch_bounds__(256) void k_step(StepArgs a)
 {
 	Ctrl* C = a.ctrl;
@@ -186,7 +231,7 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
 	const Bufs in = bi ? a.buf[1] : a.buf[0];
 	const Bufs out = bi ? a.buf[0] : a.buf[1];
 	const uint32_t n_surv = C->n_surv, n_in = n_surv + C->n_se;
-	const uint32_t B = a.slots, se_limit = C->se_limit, det_cap = a.det_cap;
+	const uint32_t B = a.slots, det_cap = a.det_cap;
 	const World& W = a.W;
 	__shared__ int s_path[PATH_LEVELS * 256];
 	int* path = s_path + threadIdx.x;
@@ -195,56 +240,32 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
 	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_in; i += stride) {
 		const uint32_t slot = (i < n_surv) ? i : (B - 1u - (i - n_surv));
 		State s = load_state(in, slot);
-		bool keep = true;
+		bool keep = true, to_inel = false;
 		for (uint32_t k = 0; k < a.K; ++k) {
-			int oc = ALIVE, ev = EV_NONE, hit = -1;
-			Rng rng;
-			rng_start(rng, W.key0, W.key1, s.sid, s.step);
-			State pre = s;
-			Particle se;
-			bool se_made = false;
+			int oc = ALIVE;
 			if (s.flags & FL_PEND_DET) {
 				oc = DETECTED;
+			} else if (s.flags & FL_PEND_INEL) {
+				// an inelastic event that found no secondary slot earlier
+				s.flags &= ~FL_PEND_INEL;
+				to_inel = true;
 			} else {
-				if (s.flags & FL_PEND_INEL) {
-					// resume an inelastic event whose path draws (0, 1) were
-					// taken when it was sampled
-					s.flags &= ~FL_PEND_INEL;
-					ev = EV_INELASTIC;
-					rng_seek(rng, 2);
-				} else {
-					oc = step_init(W, s, rng, ev, hit, path, 256);
-					++my_steps;
+				int ev = EV_NONE, hit = -1;
+				Rng rng;
+				rng_start(rng, W.key0, W.key1, s.sid, s.step);
+				oc = step_init(W, s, rng, ev, hit, path, 256);
+				++my_steps;
+				if (oc == ALIVE) {
+					if (ev == EV_INELASTIC) {
+						to_inel = true;
+					} else {
+						Particle unused;
+						bool made = false;
+						oc = step_event(W, s, rng, ev, hit, unused, made);
+					}
 				}
-				pre = s;
-				if (oc == ALIVE)
-					oc = step_event(W, s, rng, ev, hit, se, se_made);
 			}
 			bool stop = false;
-			const uint32_t sslot = wave_alloc(&C->se_ctr, se_made);
-			if (se_made) {
-				State ns;
-				ns.p = se;
-				ns.mat = s.mat;
-				ns.last_tri = -1;
-				ns.tag = s.tag;
-				ns.step = 0;
-				ns.flags = 0;
-				ns.sid = se_sid(s.sid, s.step);
-				const bool spill = sslot >= se_limit;
-				const uint32_t pslot = wave_alloc(&C->spill_ctr, spill);
-				if (!spill) {
-					store_state(out, B - 1u - sslot, ns);
-				} else if (pslot < a.spill_cap) {
-					store_state(a.spill, pslot, ns);
-				} else {
-					// no room anywhere: keep the state before the event and
-					// retry it in a later launch
-					s = pre;
-					s.flags |= FL_PEND_INEL;
-					stop = true;
-				}
-			}
 			const bool det = (oc == DETECTED);
 			const uint32_t dslot = wave_alloc(&C->det_ctr, det);
 			if (det) {
@@ -264,20 +285,52 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
 			}
 			if (oc == TERMINATED)
 				keep = false;
-			if (!keep || stop)
+			if (!keep || stop || to_inel)
 				break;
 			s.step++;
 		}
-		const uint32_t oslot = wave_alloc(&C->surv_ctr, keep);
-		if (keep)
+		const bool queue = keep && to_inel;
+		const uint32_t qslot = wave_alloc(&C->inel_ctr, queue);
+		if (queue)
+			store_state(a.inel, qslot, s);
+		const bool survive = keep && !to_inel;
+		const uint32_t oslot = wave_alloc(&C->surv_ctr, survive);
+		if (survive)
 			store_state(out, oslot, s);
 	}
-	// particle-step count: wave reduction, one atomic per wave
-	uint32_t v = my_steps;
-	for (int off = 32; off > 0; off >>= 1)
-		v += (uint32_t)__shfl_xor((int)v, off);
-	if (__lane_id() == 0 && v)
-		atomicAdd((unsigned long long*)&C->steps, (unsigned long long)v);
+	count_steps(C, my_steps);
+}
+
+// The inelastic events queued by k_step (drivers/gpu/gpu_driver.inl:505-540):
+// forget the last triangle, run the inelastic model from draw 2 of the step's
+// stream, place the secondary, and keep the particle for the next launch.
+__global__ __launch_bounds__(256) void k_inel(StepArgs a)
+{
+	Ctrl* C = a.ctrl;
+	const uint32_t bi = C->buf_in;
+	const Bufs out = bi ? a.buf[0] : a.buf[1];
+	const uint32_t n = C->inel_ctr, se_limit = C->se_limit;
+	const World& W = a.W;
+	const uint32_t stride = gridDim.x * blockDim.x;
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+		State s = load_state(a.inel, i);
+		const State pre = s;
+		Rng rng;
+		rng_start(rng, W.key0, W.key1, s.sid, s.step);
+		rng_seek(rng, 2);
+		Particle se;
+		bool made = false;
+		step_event(W, s, rng, EV_INELASTIC, -1, se, made);
+		if (place_secondary(a, C, out, se_limit, made, se, s)) {
+			s.step++;
+		} else {
+			// no room anywhere: retry the event from the same draws later
+			s = pre;
+			s.flags |= FL_PEND_INEL;
+		}
+		const uint32_t oslot = wave_alloc(&C->surv_ctr, true);
+		store_state(out, oslot, s);
+	}
 }
 
 __global__ void k_reset_det(Ctrl* c)
@@ -408,6 +461,8 @@ struct nbl_driver {
 	Bufs spill;
 	void* spill_mem;
 	uint32_t spill_cap;
+	Bufs inel;
+	void* inel_mem;
 	Ctrl* ctrl;
 	Ctrl* h_ctrl;
 	nbl_detected* det;
@@ -729,6 +784,7 @@ int nbl_driver_destroy(nbl_driver* d)
 	hipFree(d->buf_mem[0]);
 	hipFree(d->buf_mem[1]);
 	hipFree(d->spill_mem);
+	hipFree(d->inel_mem);
 	hipFree(d->ctrl);
 	hipHostFree(d->h_ctrl);
 	hipFree(d->det);
@@ -777,6 +833,7 @@ int nbl_driver_create(nbl_device* dev, uint32_t capacity, const nbl_geometry* ge
 	if ((rc = alloc_bufs(d->buf[1], d->buf_mem[1], d->slots))) return cleanup(rc);
 	d->spill_cap = std::max<uint32_t>(1u << 20, std::min<uint64_t>(4ull * d->slots, 1ull << 28));
 	if ((rc = alloc_bufs(d->spill, d->spill_mem, d->spill_cap))) return cleanup(rc);
+	if ((rc = alloc_bufs(d->inel, d->inel_mem, d->slots))) return cleanup(rc);
 	if ((rc = dev_alloc(&d->ctrl, 1))) return cleanup(rc);
 	if (hipHostMalloc((void**)&d->h_ctrl, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess)
 		return cleanup(fail(NBL_ERR_ALLOC, "hipHostMalloc failed"));
@@ -858,6 +915,7 @@ int nbl_driver_iterate(nbl_driver* d, uint32_t n_iter)
 	a.buf[1] = d->buf[1];
 	a.spill = d->spill;
 	a.spill_cap = d->spill_cap;
+	a.inel = d->inel;
 	a.ctrl = d->ctrl;
 	a.det = d->det;
 	a.det_cap = d->det_cap;
@@ -881,6 +939,7 @@ int nbl_driver_iterate(nbl_driver* d, uint32_t n_iter)
 			HIP_OK(hipEventRecord(ev.first, d->stream));
 		}
 		hipLaunchKernelGGL(k_step, dim3(g_step), dim3(256), 0, d->stream, a);
+		hipLaunchKernelGGL(k_inel, dim3(g_step), dim3(256), 0, d->stream, a);
 		if (d->timing) {
 			HIP_OK(hipEventRecord(ev.second, d->stream));
 			d->ev_used.push_back(ev);
