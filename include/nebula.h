@@ -163,6 +163,11 @@ int nbl_load_pri_file(const char* path, const float aabb_min[3], const float aab
  * the first prescan_size records with a default-seeded minstd_rand0 engine. */
 int nbl_prescan_shuffle(nbl_record36* records, int64_t n, int64_t prescan_size);
 
+/* nbl::sort_pri_file (io/load_pri_file.cpp:186-229): reorder records by the
+ * Morton code of their position over the primaries' bounding box (the
+ * nebula_gpu --sort-primaries option). */
+int nbl_sort_primaries(nbl_record36* records, int64_t n);
+
 /* octree_builder::octree_root + linearized_octree
  * (geometry/octree/octree_builder.cpp:171-352).  Outputs are allocated by the
  * library; release with nbl_free. */

@@ -111,6 +111,7 @@ _SIGS = {
     "nbl_load_pri_file": (C.c_int, [C.c_char_p, _p(C.c_float), _p(C.c_float), C.c_float,
                                     _p(C.c_void_p), _p(C.c_int64), _p(C.c_int64)]),
     "nbl_prescan_shuffle": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64]),
+    "nbl_sort_primaries": (C.c_int, [C.c_void_p, C.c_int64]),
     "nbl_octree_build": (C.c_int, [C.c_void_p, C.c_int32, _p(C.c_void_p), _p(C.c_int32),
                                    _p(C.c_void_p), _p(C.c_int32), _p(C.c_float), _p(C.c_float)]),
     "nbl_material_prepare": (C.c_int, [_p(RawMaterial), _p(_p(MaterialHost))]),

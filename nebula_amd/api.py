@@ -106,6 +106,20 @@ def load_pri_file(path: str, aabb_min, aabb_max, max_E: float):
     return rec, list(w)
 
 
+def prescan_shuffle(records: np.ndarray, prescan_size: int) -> np.ndarray:
+    """nbl::prescan_shuffle (io/load_pri_file.cpp:231-244) on a copy."""
+    rec = np.ascontiguousarray(records, dtype=PRI_DTYPE).copy()
+    check(lib().nbl_prescan_shuffle(ptr(rec), len(rec), int(prescan_size)))
+    return rec
+
+
+def sort_primaries(records: np.ndarray) -> np.ndarray:
+    """nbl::sort_pri_file (io/load_pri_file.cpp:186-229) on a copy."""
+    rec = np.ascontiguousarray(records, dtype=PRI_DTYPE).copy()
+    check(lib().nbl_sort_primaries(ptr(rec), len(rec)))
+    return rec
+
+
 class Device:
     """One GPU (cudaSetDevice + stream of gpu_main.cu:97)."""
 

@@ -4,6 +4,7 @@
 #include "host.hpp"
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -624,6 +625,62 @@ void prescan_shuffle(nbl_record36* rec, int64_t n, int64_t prescan_size)
 	}
 }
 
+// sort_pri_file (io/load_pri_file.cpp:29-40,186-229): order primaries by the
+// Morton code of their position quantised to 16 bits per axis over the
+// primaries' bounding box.  The reference sorts an index permutation with
+// std::sort (not stable) and a comparator that evaluates both codes; sorting
+// the same permutation with the same comparator outcomes gives the same order,
+// so the codes are computed once.  A float->uint16 conversion is made through
+// int32, as x86-64 code does (an axis of zero extent gives 0/0: NaN -> 0).
+static uint64_t morton16(uint16_t x, uint16_t y, uint16_t z)
+{
+	uint64_t m = 0;
+	for (unsigned i = 0; i < 16; ++i) {
+		m |= (x & (uint64_t(1) << i)) << (2 * i + 0);
+		m |= (y & (uint64_t(1) << i)) << (2 * i + 1);
+		m |= (z & (uint64_t(1) << i)) << (2 * i + 2);
+	}
+	return m;
+}
+
+static uint16_t quantise16(float v)
+{
+	return (uint16_t)(int32_t)v;
+}
+
+void sort_primaries(nbl_record36* rec, int64_t n)
+{
+	if (n <= 0)
+		return;
+	float vmin[3], vmax[3];
+	for (int k = 0; k < 3; ++k)
+		vmin[k] = vmax[k] = rec[0].pos[k];
+	for (int64_t i = 0; i < n; ++i)
+		for (int k = 0; k < 3; ++k) {
+			vmin[k] = std::min(vmin[k], rec[i].pos[k]);
+			vmax[k] = std::max(vmax[k], rec[i].pos[k]);
+		}
+	float size[3];
+	for (int k = 0; k < 3; ++k)
+		size[k] = vmax[k] - vmin[k];
+	const float qmax = (float)std::numeric_limits<uint16_t>::max();
+	std::vector<uint64_t> code((size_t)n);
+	for (int64_t i = 0; i < n; ++i) {
+		uint16_t q[3];
+		for (int k = 0; k < 3; ++k)
+			q[k] = quantise16((rec[i].pos[k] - vmin[k]) / size[k] * qmax);
+		code[(size_t)i] = morton16(q[0], q[1], q[2]);
+	}
+	std::vector<size_t> perm((size_t)n);
+	for (size_t i = 0; i < perm.size(); ++i)
+		perm[i] = i;
+	std::sort(perm.begin(), perm.end(), [&](size_t a, size_t b) { return code[a] < code[b]; });
+	std::vector<nbl_record36> sorted((size_t)n);
+	for (size_t i = 0; i < perm.size(); ++i)
+		sorted[i] = rec[perm[i]];
+	std::memcpy(rec, sorted.data(), sizeof(nbl_record36) * (size_t)n);
+}
+
 } // namespace nbl
 
 // =================================================================== C ABI ==
@@ -676,6 +733,15 @@ int nbl_prescan_shuffle(nbl_record36* records, int64_t n, int64_t prescan_size)
 	if (!records || n < 0 || prescan_size < 0)
 		return fail(NBL_ERR_ARG, "invalid argument");
 	prescan_shuffle(records, n, prescan_size);
+	return NBL_OK;
+}
+
+int nbl_sort_primaries(nbl_record36* records, int64_t n)
+{
+	clear_error();
+	if ((!records && n) || n < 0)
+		return fail(NBL_ERR_ARG, "invalid argument");
+	sort_primaries(records, n);
 	return NBL_OK;
 }
 

@@ -23,5 +23,6 @@ int load_tri_file(const char* path, std::vector<nbl_triangle44>& out);
 int load_pri_file(const char* path, const float mn[3], const float mx[3], float max_E,
 	std::vector<nbl_record36>& out, int64_t warnings[4]);
 void prescan_shuffle(nbl_record36* rec, int64_t n, int64_t prescan_size);
+void sort_primaries(nbl_record36* rec, int64_t n);
 
 } // namespace nbl

@@ -10,6 +10,9 @@
 //             its own mt19937(seed + i)
 //   simulate  <tris.bin> <penn|kieft> <seed> <pri.bin> <out.bin> <mat>...
 //             simple_cpu_driver over all primaries, one thread
+//   pri       <in.pri> <prescan_size> <sort 0|1> <out.bin>
+//             load_pri_file, optional sort_pri_file, prescan_shuffle, in the
+//             order of gpu_main.cu:364-367; out = the 36-byte records
 // Binary layouts are little-endian and documented beside each reader.
 // Access to private members (tables, particle vector) is needed to dump
 // them; the reference files themselves are untouched.
@@ -27,6 +30,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -285,6 +289,23 @@ template <typename L> static int cmd_simulate(char** a, int nmat)
 	return 0;
 }
 
+static int cmd_pri(char** a)
+{
+	const float big = std::numeric_limits<float>::max();
+	std::vector<particle> prim;
+	std::vector<int2> pix;
+	std::tie(prim, pix) = nbl::load_pri_file(a[0], { -big, -big, -big }, { big, big, big }, big);
+	if (std::stoi(a[2]))
+		nbl::sort_pri_file(prim, pix);
+	nbl::prescan_shuffle(prim, pix, (size_t)std::stoull(a[1]));
+	Out o(a[3]);
+	for (size_t i = 0; i < prim.size(); ++i) {
+		o.put(prim[i]);
+		o.put(pix[i]);
+	}
+	return 0;
+}
+
 int main(int argc, char** argv)
 {
 	if (argc < 2) {
@@ -298,6 +319,7 @@ int main(int argc, char** argv)
 		if (c == "octree") return cmd_octree(a);
 		if (c == "propagate") return cmd_propagate(a);
 		if (c == "intersect") return cmd_intersect(a);
+		if (c == "pri") return cmd_pri(a);
 		const bool penn = na > 1 && std::string(a[1]) == "penn";
 		if (c == "material") return penn ? cmd_material<penn_list>(a) : cmd_material<kieft_list>(a);
 		if (c == "probe") return penn ? cmd_probe<penn_list>(a) : cmd_probe<kieft_list>(a);

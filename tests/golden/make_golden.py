@@ -221,6 +221,29 @@ def main():
         fx[f"sim_{model}_pri"] = pri
         fx[f"sim_{model}_det"] = det.copy()
         print(f"simulate {model}: {len(pri)} primaries, {n} detected")
+    # ------------------------------------- primary order: sort + prescan shuffle
+    # 3000 primaries on a coarse lattice (many equal Morton codes, so the
+    # unstable std::sort order matters) at one height (a zero-extent axis).
+    r = np.random.default_rng(31)
+    n = 3000
+    rec = np.zeros(n, W.PRI_DTYPE)
+    rec["pos"][:, 0] = 2.0 * r.integers(-4, 5, n)
+    rec["pos"][:, 1] = 2.0 * r.integers(-4, 5, n) + 0.5 * (r.random(n) < 0.3)
+    rec["pos"][:, 2] = 1.0
+    rec["dir"][:, 2] = -1.0
+    rec["E"] = r.uniform(100, 1000, n).astype(np.float32)
+    rec["px"] = np.arange(n)
+    rec["py"] = -np.arange(n)
+    pf = os.path.join(tmp, "order.pri")
+    rec.tofile(pf)
+    fx["pri_order_in"] = rec
+    for srt in (0, 1):
+        of = os.path.join(tmp, f"order{srt}.out")
+        run("pri", pf, 1000, srt, of)
+        out = np.fromfile(of, W.PRI_DTYPE)
+        assert len(out) == n
+        fx[f"pri_order_sort{srt}"] = out["px"].astype(np.int32)
+    print("primary order: sort + shuffle vectors")
     np.savez_compressed(os.path.join(OUT, "reference_vectors.npz"), **fx)
     print("wrote", os.path.join(OUT, "reference_vectors.npz"),
           os.path.getsize(os.path.join(OUT, "reference_vectors.npz")), "bytes")

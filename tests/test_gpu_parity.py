@@ -5,11 +5,9 @@ and distance), table lookups, and whole steps (event, outcome, new state,
 secondary electron) on identical inputs and Philox streams.  Whole cascades
 are compared record by record: every electron's trajectory depends only on
 its own stream, so the GPU's detected electrons must equal the oracle's
-(same stream ids, same bits) whatever the scheduling.  Transcendentals on
-both sides are evaluated in double and rounded to float; a double result
-that straddles a float rounding boundary differently in the two math
-libraries (probability ~2^-29 per call) is the only allowed source of a
-difference, so the tests allow a tiny mismatch count and report it.
+(same stream ids, same bits) whatever the scheduling.  Both sides evaluate
+transcendentals with the same float-only routines and no FMA contraction,
+so every comparison is exact: no mismatch is tolerated.
 """
 import numpy as np
 import pytest
@@ -121,7 +119,7 @@ def test_step_bit_exact(dev, model):
     print("events:", np.bincount(g_ev, minlength=4), "se made:", g_made.sum())
     for k in ("pos", "dir", "E"):
         mism = np.any(_bits(g_st[k]).reshape(n, -1) != _bits(o_st[k]).reshape(n, -1), axis=1)
-        assert mism.sum() <= max(2, n // 5000), (k, mism.sum())
+        assert mism.sum() == 0, (k, mism.sum())
     for k in ("mat", "last_tri", "step"):
         assert np.array_equal(g_st[k], o_st[k]), k
     m = g_made.astype(bool)
@@ -129,7 +127,7 @@ def test_step_bit_exact(dev, model):
     assert np.array_equal(g_se["sid"][m], o_se["sid"][m])
     for k in ("pos", "dir", "E"):
         mism = np.any(_bits(g_se[k][m]).reshape(m.sum(), -1) != _bits(o_se[k][m]).reshape(m.sum(), -1), axis=1)
-        assert mism.sum() <= max(2, n // 5000), (k, mism.sum())
+        assert mism.sum() == 0, (k, mism.sum())
 
 
 def _compare_detected(gpu, orc):
@@ -168,4 +166,4 @@ def test_cascades_match_oracle(dev, capacity, K):
     n_common, n_same, n_only = _compare_detected(det_g, det_o)
     print(f"detected gpu={len(det_g)} oracle={len(det_o)} common={n_common} same={n_same} only={n_only}")
     assert len(det_g) > 50
-    assert n_only <= 2 and n_common - n_same <= 2
+    assert n_only == 0 and n_common == n_same == len(det_o) == len(det_g)
