@@ -137,7 +137,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the run, copy 1 GiB device-to-device with a plain copy kernel "
                          "(reference point for the FETCH_SIZE / WRITE_SIZE counters)")
-    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "k_step_traffic.json"),
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "step_pair_traffic.json"),
                     help="measured HBM bytes per k_step + k_inel launch pair (from rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
