@@ -1143,6 +1143,8 @@ int orc_material_create(const orc_raw_material* r, orc_material* m)
 	int rc;
 	/* core/material.inl:13-18 */
 	m->barrier = (float)(r->barrier / 1.0);
+	m->el_scale = 1.0f;
+	m->in_scale = 1.0f;
 	m->inel_model = r->inel_model;
 
 	/* kieft_elastic::create (elastic.h:120-166) */
@@ -1286,27 +1288,38 @@ float orc_material_max_energy(const orc_material* m)
 }
 
 /* -------------------------------------------------------------- physics -- */
-/* full_penn.h:50-57 / elastic.h:54-62 / inelastic.h:54-61 */
-static float mech_sample_path(const orc_table1d* t, const orc_particle* p, orc_rng* rng)
+/* full_penn.h:50-57 / elastic.h:54-62 / inelastic.h:54-61; the mean free
+ * path is multiplied by the mechanism's scale (exact when it is 1).  *tau
+ * receives the scaled mean free path. */
+static float mech_sample_path(const orc_table1d* t, float scale, const orc_particle* p, orc_rng* rng,
+	float* tau)
 {
 	const float imfp = m_exp(orc_t1_get(t, m_log(p->E)));
-	return rng_exponential(rng, 1.0f / imfp);
+	*tau = (1.0f / imfp) * scale;
+	return rng_exponential(rng, *tau);
 }
 
 /* core/scatter_list.h:101-107 + common/tuple.h:114-124: the elastic
  * mechanism (index 1) is sampled before the inelastic one (index 0); the
  * strictly shorter path wins. */
-void orc_sample_path(const orc_material* m, const orc_particle* p, orc_rng* rng,
-	uint8_t* type, float* distance)
+static void sample_path_tau(const orc_material* m, const orc_particle* p, orc_rng* rng,
+	uint8_t* type, float* distance, float* tau_el, float* tau_in)
 {
 	uint8_t ty = 0;
 	float dist = INFINITY;
-	const float pe = mech_sample_path(&m->el_imfp, p, rng);
+	const float pe = mech_sample_path(&m->el_imfp, m->el_scale, p, rng, tau_el);
 	if (pe < dist) { ty = 2; dist = pe; }
-	const float pi = mech_sample_path(&m->in_imfp, p, rng);
+	const float pi = mech_sample_path(&m->in_imfp, m->in_scale, p, rng, tau_in);
 	if (pi < dist) { ty = 1; dist = pi; }
 	*type = ty;
 	*distance = dist;
+}
+
+void orc_sample_path(const orc_material* m, const orc_particle* p, orc_rng* rng,
+	uint8_t* type, float* distance)
+{
+	float te, ti;
+	sample_path_tau(m, p, rng, type, distance, &te, &ti);
 }
 
 /* physics/kieft/elastic.h:67-115 */
@@ -1528,9 +1541,19 @@ int orc_kieft_execute(const orc_material* m, const orc_physics_opts* o,
 	return made;
 }
 
+static void add_barrier_score(const orc_score_map* map, double* score, int32_t mi, int32_t mo, float g)
+{
+	if (!map || g == 0.0f)
+		return;
+	if (mo >= 0 && mo < ORC_MAX_MATERIALS && map->pidx[mo][ORC_P_BARRIER] >= 0)
+		score[map->pidx[mo][ORC_P_BARRIER]] += (double)g;
+	if (mi >= 0 && mi < ORC_MAX_MATERIALS && map->pidx[mi][ORC_P_BARRIER] >= 0)
+		score[map->pidx[mi][ORC_P_BARRIER]] -= (double)g;
+}
+
 /* physics/boundary_intersect.h:38-155 */
-int orc_boundary_execute(const orc_world* w, const orc_triangle* tri,
-	orc_particle* p, int32_t* mat, orc_rng* rng)
+static int boundary_execute_scored(const orc_world* w, const orc_triangle* tri,
+	orc_particle* p, int32_t* mat, orc_rng* rng, const orc_score_map* map, double* score)
 {
 	orc_particle tp = *p;
 	const orc_vec3 nd = normalised(tp.dir);
@@ -1564,9 +1587,12 @@ int orc_boundary_execute(const orc_world* w, const orc_triangle* tri,
 	if (mo >= 0) dU += w->materials[mo].barrier;
 
 	if (tp.E * cos_theta * cos_theta + dU > 0.0f) {
-		const float s = sqrtf(1.0f + dU / (tp.E * cos_theta * cos_theta));
+		const float ec2 = tp.E * cos_theta * cos_theta;
+		const float s = sqrtf(1.0f + dU / ec2);
 		const float T = w->opts.bi_quantum_transmission ? 4.0f * s / ((1.0f + s) * (1.0f + s)) : 1.0f;
 		if (orc_rng_unit(rng) < T) {
+			if (w->opts.bi_quantum_transmission && T < 1.0f)
+				add_barrier_score(map, score, mi, mo, (1.0f - s) / (2.0f * s * s * (1.0f + s) * ec2));
 			if (w->opts.bi_interface_refraction)
 				tp.dir = v_add(v_sub(nd, v_mul(n, cos_theta)), v_mul(s_mul(s, n), cos_theta));
 			tp.E += dU;
@@ -1574,22 +1600,37 @@ int orc_boundary_execute(const orc_world* w, const orc_triangle* tri,
 			*p = tp;
 			return ORC_ALIVE;
 		}
+		if (w->opts.bi_quantum_transmission && T < 1.0f)
+			add_barrier_score(map, score, mi, mo, 2.0f / (s * dU));
 	}
 	if (w->opts.bi_interface_absorption) {
-		if (dU < 0.0f && orc_rng_unit(rng) < m_exp(1.0f + 0.5f * tp.E / dU))
-			return ORC_TERMINATED;
+		if (dU < 0.0f) {
+			const float P = m_exp(1.0f + 0.5f * tp.E / dU);
+			const int absorbed = orc_rng_unit(rng) < P;
+			if (P < 1.0f)
+				add_barrier_score(map, score, mi, mo, absorbed ? -0.5f * tp.E / (dU * dU)
+					: P * 0.5f * tp.E / (dU * dU * (1.0f - P)));
+			if (absorbed)
+				return ORC_TERMINATED;
+		}
 	}
 	tp.dir = v_sub(nd, v_mul(s_mul(2.0f, n), cos_theta));
 	*p = tp;
 	return ORC_ALIVE;
 }
 
+int orc_boundary_execute(const orc_world* w, const orc_triangle* tri,
+	orc_particle* p, int32_t* mat, orc_rng* rng)
+{
+	return boundary_execute_scored(w, tri, p, mat, rng, NULL, NULL);
+}
+
 /* --------------------------------------------------------------- a step -- */
 /* init (drivers/cpu/cpu_driver.inl:74-126 / drivers/gpu/gpu_driver.inl:
  * 403-480) followed by intersect (:128-139 / :482-503) and scatter
  * (:141-160 / :505-566). */
-int orc_step(const orc_world* w, int semantics, orc_state* s, orc_rng* rng,
-	orc_state* se, int* se_made, int* event)
+int orc_step_scored(const orc_world* w, int semantics, orc_state* s, orc_rng* rng,
+	orc_state* se, int* se_made, int* event, const orc_score_map* map, double* score)
 {
 	*se_made = 0;
 	*event = ORC_EV_NONE;
@@ -1598,11 +1639,12 @@ int orc_step(const orc_world* w, int semantics, orc_state* s, orc_rng* rng,
 
 	uint8_t type = 0;
 	float distance = w->geom.max_extent;
+	float tau_el = 0.0f, tau_in = 0.0f;
 	if (s->mat >= 0) {
 		const orc_material* m = &w->materials[s->mat];
 		if (s->p.E < m->barrier + w->energy_threshold)
 			return ORC_TERMINATED;
-		orc_sample_path(m, &s->p, rng, &type, &distance);
+		sample_path_tau(m, &s->p, rng, &type, &distance, &tau_el, &tau_in);
 	}
 
 	const orc_vec3 nd = normalised(s->p.dir);
@@ -1618,9 +1660,19 @@ int orc_step(const orc_world* w, int semantics, orc_state* s, orc_rng* rng,
 		s->last_tri = hit;
 		*event = ORC_EV_INTERSECT;
 	}
+	if (map && s->mat >= 0 && s->mat < ORC_MAX_MATERIALS && type != 0) {
+		/* free-path score: distance travelled d, rates 1/tau */
+		const orc_material* m = &w->materials[s->mat];
+		const float d = (hit < 0) ? distance : hit_dist;
+		const int pe = map->pidx[s->mat][ORC_P_EL_SCALE], pi = map->pidx[s->mat][ORC_P_IN_SCALE];
+		if (pe >= 0)
+			score[pe] += (double)((d / tau_el - ((*event == ORC_EV_ELASTIC) ? 1.0f : 0.0f)) / m->el_scale);
+		if (pi >= 0)
+			score[pi] += (double)((d / tau_in - ((*event == ORC_EV_INELASTIC) ? 1.0f : 0.0f)) / m->in_scale);
+	}
 
 	if (*event == ORC_EV_INTERSECT)
-		return orc_boundary_execute(w, &w->geom.triangles[hit], &s->p, &s->mat, rng);
+		return boundary_execute_scored(w, &w->geom.triangles[hit], &s->p, &s->mat, rng, map, score);
 	if (*event == ORC_EV_NONE)
 		return ORC_ALIVE;
 
@@ -1644,6 +1696,12 @@ int orc_step(const orc_world* w, int semantics, orc_state* s, orc_rng* rng,
 		*se_made = 1;
 	}
 	return ORC_ALIVE;
+}
+
+int orc_step(const orc_world* w, int semantics, orc_state* s, orc_rng* rng,
+	orc_state* se, int* se_made, int* event)
+{
+	return orc_step_scored(w, semantics, s, rng, se, se_made, event, NULL, NULL);
 }
 
 /* --------------------------------------------------------------- drivers -- */
@@ -1713,21 +1771,28 @@ typedef struct {
 	size_t n;
 	uint64_t seed, sid_base;
 	uint32_t tag_base;
+	const orc_score_map* map;     /* NULL: no scores */
 	atomic_size_t next;
 	orc_detected* out;
+	double* out_scores;
 	size_t cap;
 	atomic_llong nout;
 	atomic_ullong steps;
 	atomic_int overflow;
 } philox_job;
 
+/* One entry of a worker's depth-first stack: a particle and its score. */
+typedef struct { orc_state s; double score[ORC_MAX_PARAMS]; } scored_state;
+
 static void* philox_worker(void* arg)
 {
 	philox_job* job = (philox_job*)arg;
+	const orc_score_map* map = job->map;
+	const int np = map ? map->n_params : 0;
 	orc_rng rng;
 	orc_rng_init_philox(&rng, job->seed);
 	size_t scap = 256;
-	orc_state* stack = (orc_state*)malloc(sizeof(orc_state) * scap);
+	scored_state* stack = (scored_state*)malloc(sizeof(scored_state) * scap);
 	uint64_t steps = 0;
 	for (;;) {
 		const size_t i = atomic_fetch_add(&job->next, 64);
@@ -1735,27 +1800,31 @@ static void* philox_worker(void* arg)
 		const size_t iend = i + 64 < job->n ? i + 64 : job->n;
 		for (size_t k = i; k < iend; ++k) {
 			size_t sn = 1;
-			memset(&stack[0], 0, sizeof(orc_state));
-			stack[0].p = job->primaries[k];
-			stack[0].mat = ORC_VACUUM;
-			stack[0].last_tri = -1;
-			stack[0].tag = job->tag_base + (uint32_t)k;
-			stack[0].sid = job->sid_base + k;
+			memset(&stack[0], 0, sizeof(scored_state));
+			stack[0].s.p = job->primaries[k];
+			stack[0].s.mat = ORC_VACUUM;
+			stack[0].s.last_tri = -1;
+			stack[0].s.tag = job->tag_base + (uint32_t)k;
+			stack[0].s.sid = job->sid_base + k;
 			while (sn > 0) {
-				orc_state s = stack[--sn];
+				scored_state cur = stack[--sn];
+				orc_state s = cur.s;
 				for (;;) {
 					orc_state se;
 					int made, ev;
 					orc_rng_set_stream(&rng, s.sid, s.step);
 					++steps;
-					const int st = orc_step(job->w, 1, &s, &rng, &se, &made, &ev);
+					const int st = orc_step_scored(job->w, 1, &s, &rng, &se, &made, &ev, map, cur.score);
 					s.step++;
 					if (made) {
 						if (sn == scap) {
 							scap *= 2;
-							stack = (orc_state*)realloc(stack, sizeof(orc_state) * scap);
+							stack = (scored_state*)realloc(stack, sizeof(scored_state) * scap);
 						}
-						stack[sn++] = se;
+						/* a secondary inherits its parent's score so far */
+						stack[sn].s = se;
+						memcpy(stack[sn].score, cur.score, sizeof(cur.score));
+						++sn;
 					}
 					if (st == ORC_ALIVE) continue;
 					if (st == ORC_DETECTED) {
@@ -1764,6 +1833,8 @@ static void* philox_worker(void* arg)
 							job->out[slot].p = s.p;
 							job->out[slot].tag = s.tag;
 							job->out[slot].sid = s.sid;
+							for (int q = 0; q < np; ++q)
+								job->out_scores[(size_t)slot * (size_t)np + (size_t)q] = cur.score[q];
 						} else {
 							atomic_store(&job->overflow, 1);
 						}
@@ -1778,15 +1849,18 @@ static void* philox_worker(void* arg)
 	return NULL;
 }
 
-int64_t orc_run_philox(const orc_world* w, const orc_particle* primaries,
+int64_t orc_run_philox_scored(const orc_world* w, const orc_particle* primaries,
 	size_t n, uint64_t seed, uint64_t sid_base, uint32_t tag_base, int threads,
-	orc_detected* out, size_t cap, uint64_t* n_steps)
+	const orc_score_map* map, orc_detected* out, double* out_scores, size_t cap,
+	uint64_t* n_steps)
 {
+	if (map && (map->n_params < 0 || map->n_params > ORC_MAX_PARAMS))
+		return -2;
 	philox_job job;
 	memset(&job, 0, sizeof(job));
 	job.w = w; job.primaries = primaries; job.n = n;
 	job.seed = seed; job.sid_base = sid_base; job.tag_base = tag_base;
-	job.out = out; job.cap = cap;
+	job.map = map; job.out = out; job.out_scores = out_scores; job.cap = cap;
 	atomic_init(&job.next, 0);
 	atomic_init(&job.nout, 0);
 	atomic_init(&job.steps, 0);
@@ -1801,6 +1875,14 @@ int64_t orc_run_philox(const orc_world* w, const orc_particle* primaries,
 	if (n_steps) *n_steps = atomic_load(&job.steps);
 	if (atomic_load(&job.overflow)) return -1;
 	return (int64_t)atomic_load(&job.nout);
+}
+
+int64_t orc_run_philox(const orc_world* w, const orc_particle* primaries,
+	size_t n, uint64_t seed, uint64_t sid_base, uint32_t tag_base, int threads,
+	orc_detected* out, size_t cap, uint64_t* n_steps)
+{
+	return orc_run_philox_scored(w, primaries, n, seed, sid_base, tag_base, threads, NULL, out, NULL,
+		cap, n_steps);
 }
 
 void orc_pf_eval_batch(int fn, const float* a, const float* b, float* out, size_t n)

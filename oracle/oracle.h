@@ -85,6 +85,10 @@ typedef struct {
 	orc_table3d pn_ion;            /* penn: binding vs (log E, log(omega/E), P) */
 	orc_table2d ki_ion;            /* kieft: binding vs (log omega', P) */
 	float fermi, band_gap;
+	/* Model parameters of the derivative estimator (no reference
+	 * counterpart): sampled elastic / inelastic free paths are multiplied by
+	 * these mean-free-path scales (1 after create). */
+	float el_scale, in_scale;
 } orc_material;
 
 /* Physics options: template flags of physics_config.h:21-38 made runtime. */
@@ -257,6 +261,38 @@ int orc_boundary_execute(const orc_world* w, const orc_triangle* tri,
 int orc_step(const orc_world* w, int semantics, orc_state* s, orc_rng* rng,
 	orc_state* se, int* se_made, int* event);
 
+/* ------------------------------------------- likelihood-ratio scores -- */
+/* Parameters whose derivative is estimated: per material, the elastic and
+ * inelastic mean-free-path scales and the barrier (work function + Fermi
+ * energy).  pidx[m][k] is the parameter index of (material m, kind k) or -1.
+ * A particle's score is the sum over its own steps, and over its ancestors'
+ * steps up to its creation, of d log p(step outcome) / d theta:
+ *   free path in material m, rates r_j = 1/tau_j (tau_j = scaled mean free
+ *   path), travelled distance d, event j or none (boundary first):
+ *       d/d s_j = (d * r_j - [event == j]) / s_j
+ *   boundary transmission with T = 4s/(1+s)^2, s = sqrt(1 + dU/(E c^2)),
+ *   dU = b_out - b_in:
+ *       transmitted: d log T / d dU = (1-s) / (2 s^2 (1+s) E c^2)
+ *       reflected:   d log(1-T) / d dU = 2 / (s dU)
+ *   interface absorption (if enabled), P = exp(1 + E/(2 dU)):
+ *       absorbed:    d log P / d dU = -E / (2 dU^2)
+ *       not:         P E / (2 dU^2 (1-P))
+ *   and d dU / d b_out = 1, d dU / d b_in = -1.
+ * The kinematic effects of a barrier (energy shift, refraction) are not
+ * random choices and carry no score. */
+#define ORC_MAX_PARAMS 8
+#define ORC_MAX_MATERIALS 16
+enum { ORC_P_EL_SCALE = 0, ORC_P_IN_SCALE = 1, ORC_P_BARRIER = 2 };
+typedef struct {
+	int n_params;
+	int pidx[ORC_MAX_MATERIALS][3];
+} orc_score_map;
+
+/* orc_step plus score accumulation into score[n_params] (map may be NULL:
+ * then it is orc_step). */
+int orc_step_scored(const orc_world* w, int semantics, orc_state* s, orc_rng* rng,
+	orc_state* se, int* se_made, int* event, const orc_score_map* map, double* score);
+
 /* ----------------------------------------------------------------- driver -- */
 typedef struct {
 	orc_particle p;
@@ -278,6 +314,14 @@ int64_t orc_run_reference(const orc_world* w, const orc_particle* primaries,
 int64_t orc_run_philox(const orc_world* w, const orc_particle* primaries,
 	size_t n, uint64_t seed, uint64_t sid_base, uint32_t tag_base, int threads,
 	orc_detected* out, size_t cap, uint64_t* n_steps);
+
+/* orc_run_philox with scores: out_scores[i * map->n_params + k] is the
+ * score of detected record i for parameter k (secondaries inherit their
+ * parent's score at creation). */
+int64_t orc_run_philox_scored(const orc_world* w, const orc_particle* primaries,
+	size_t n, uint64_t seed, uint64_t sid_base, uint32_t tag_base, int threads,
+	const orc_score_map* map, orc_detected* out, double* out_scores, size_t cap,
+	uint64_t* n_steps);
 
 #ifdef __cplusplus
 }

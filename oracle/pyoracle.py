@@ -47,7 +47,28 @@ class T3(C.Structure):
 class Material(C.Structure):
     _fields_ = [("barrier", C.c_float), ("el_imfp", T1), ("el_icdf", T2), ("phonon_loss", C.c_float),
                 ("recoil_const", C.c_float), ("inel_model", C.c_int), ("in_imfp", T1), ("in_icdf", T2),
-                ("pn_q", T3), ("pn_ion", T3), ("ki_ion", T2), ("fermi", C.c_float), ("band_gap", C.c_float)]
+                ("pn_q", T3), ("pn_ion", T3), ("ki_ion", T2), ("fermi", C.c_float), ("band_gap", C.c_float),
+                ("el_scale", C.c_float), ("in_scale", C.c_float)]
+
+
+class ScoreMap(C.Structure):
+    _fields_ = [("n_params", C.c_int), ("pidx", (C.c_int * 3) * 16)]
+
+
+P_EL_SCALE, P_IN_SCALE, P_BARRIER = 0, 1, 2
+
+
+def score_map(params):
+    """params: list of (material index, kind) with kind 0 = elastic MFP scale,
+    1 = inelastic MFP scale, 2 = barrier."""
+    m = ScoreMap()
+    m.n_params = len(params)
+    for a in range(16):
+        for b in range(3):
+            m.pidx[a][b] = -1
+    for i, (mat, kind) in enumerate(params):
+        m.pidx[mat][kind] = i
+    return m
 
 
 class Opts(C.Structure):
@@ -129,6 +150,13 @@ def load(kind: str = "pf"):
         L.orc_run_philox.argtypes = [C.POINTER(World), vp, C.c_size_t, C.c_uint64, C.c_uint64, C.c_uint32,
                                      C.c_int, vp, C.c_size_t, C.POINTER(C.c_uint64)]
         L.orc_run_philox.restype = C.c_int64
+        L.orc_run_philox_scored.argtypes = [C.POINTER(World), vp, C.c_size_t, C.c_uint64, C.c_uint64,
+                                            C.c_uint32, C.c_int, C.POINTER(ScoreMap), vp, vp, C.c_size_t,
+                                            C.POINTER(C.c_uint64)]
+        L.orc_run_philox_scored.restype = C.c_int64
+        L.orc_step_scored.argtypes = [C.POINTER(World), C.c_int, C.POINTER(State), C.POINTER(Rng),
+                                      C.POINTER(State), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(ScoreMap), C.POINTER(C.c_double)]
         L.orc_step.argtypes = [C.POINTER(World), C.c_int, C.POINTER(State), C.POINTER(Rng), C.POINTER(State),
                                C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_rng_init_philox.argtypes = [C.POINTER(Rng), C.c_uint64]
@@ -335,6 +363,60 @@ class OracleWorld:
         self.L.orc_geometry_queries(C.byref(self.w.geom), _p(pos), _p(dir), _p(dist), _p(it), _p(im),
                                     _p(ot), _p(od), n)
         return ot, od
+
+    def set_params(self, mat: int, el_scale=None, in_scale=None, barrier=None):
+        """Model parameters of material `mat` (the world's own copy)."""
+        m = self.mat_arr[mat]
+        if el_scale is not None:
+            m.el_scale = el_scale
+        if in_scale is not None:
+            m.in_scale = in_scale
+        if barrier is not None:
+            m.barrier = barrier
+
+    def run_philox_scored(self, primaries: np.ndarray, seed: int, cap: int, params, threads=1,
+                          sid_base=0, tag_base=0):
+        """run_philox plus the likelihood-ratio score of every detected
+        record for the (material, kind) parameters `params`."""
+        p = _particles(primaries)
+        out = np.zeros(cap, DET_DTYPE)
+        sc = np.zeros((cap, max(1, len(params))), np.float64)
+        mp = score_map(params)
+        steps = C.c_uint64()
+        n = self.L.orc_run_philox_scored(C.byref(self.w), _p(p), len(p), seed, sid_base, tag_base, threads,
+                                         C.byref(mp), _p(out), _p(sc), cap, C.byref(steps))
+        if n < 0:
+            raise ValueError("output capacity exceeded")
+        return out[:n], sc[:n, :len(params)], steps.value
+
+    def step_scored(self, states: np.ndarray, seed: int, params, semantics: int = 1):
+        """One orc_step_scored per state with its Philox stream: (outcome,
+        event, new states, score increments)."""
+        n = len(states)
+        st = np.ascontiguousarray(states, STATE_DTYPE).copy()
+        oc = np.zeros(n, np.int32)
+        ev = np.zeros(n, np.int32)
+        sc = np.zeros((n, max(1, len(params))), np.float64)
+        mp = score_map(params)
+        rng = Rng()
+        self.L.orc_rng_init_philox(C.byref(rng), seed)
+        S = (State * 1)()
+        SE = (State * 1)()
+        m = C.c_int()
+        e = C.c_int()
+        row = (C.c_double * 8)()
+        for i in range(n):
+            C.memmove(S, st[i:i + 1].ctypes.data, C.sizeof(State))
+            for q in range(8):
+                row[q] = 0.0
+            self.L.orc_rng_set_stream(C.byref(rng), int(st[i]["sid"]), int(st[i]["step"]))
+            oc[i] = self.L.orc_step_scored(C.byref(self.w), semantics, S, C.byref(rng), SE, C.byref(m),
+                                           C.byref(e), C.byref(mp), row)
+            ev[i] = e.value
+            S[0].step += 1
+            C.memmove(st[i:i + 1].ctypes.data, S, C.sizeof(State))
+            sc[i] = row[:max(1, len(params))]
+        return oc, ev, st, sc[:, :len(params)]
 
     def run_reference(self, primaries: np.ndarray, seed: int, cap: int):
         """CPU-driver semantics, one mt19937 stream (the reference's simple_cpu_driver)."""
