@@ -221,10 +221,43 @@ int nbl_driver_destroy(nbl_driver* d);
 int nbl_driver_push(nbl_driver* d, const nbl_particle28* particles, const uint32_t* tags,
 	uint32_t n, uint64_t sid_base, uint32_t* pushed);
 
+/* Several scans in one driver (no counterpart: the reference runs one
+ * geometry per process).  add_world registers another geometry with its own
+ * material list (world 0 is the creation geometry) and returns its id;
+ * push_world queues primaries that track in that world.  Secondaries stay
+ * in their parent's world.  Physics flags, threshold and seed are shared;
+ * nbl_driver_set_material_params applies to world 0's materials. */
+int nbl_driver_add_world(nbl_driver* d, const nbl_geometry* geom, const nbl_material* const* mats,
+	int32_t n_mats, int32_t* world);
+int nbl_driver_push_world(nbl_driver* d, const nbl_particle28* particles, const uint32_t* tags,
+	uint32_t n, uint64_t sid_base, int32_t world, uint32_t* pushed);
+
 /* frame_size x gpu_driver::do_iteration (gpu_main.cu:209-210,
  * drivers/gpu/gpu_driver.inl:313-346).  Enqueues n_iter iterations on the
  * driver's stream without waiting. */
 int nbl_driver_iterate(nbl_driver* d, uint32_t n_iter);
+
+/* Launch configuration: steps each particle takes per k_step launch,
+ * k_step blocks (256 threads) per compute unit, and particles in flight to
+ * aim for (<= capacity; the reference's batch sizing, gpu_main.cu:167-175). */
+int nbl_driver_set_launch(nbl_driver* d, uint32_t steps_per_launch, uint32_t blocks_per_cu,
+	uint32_t target);
+
+/* Replay each nbl_driver_iterate(d, n > 1) from a captured hipGraph of n
+ * iterations (captured once per n and launch configuration). */
+int nbl_driver_set_graph(nbl_driver* d, int enable);
+
+/* The prescan autotune of gpu_main.cu:136-175 made a measured search:
+ * coordinate search over steps per launch {4,8,16,32}, blocks per compute
+ * unit {4,5,6,8} and particles in flight {capacity/4, /2, 1}, each trial
+ * `iters_per_trial` iterations of the queued primaries (real work: their
+ * detected electrons stay in the buffer for the next drain).  Leaves the
+ * fastest configuration (particle-steps/s) set on the driver. */
+typedef struct {
+	uint32_t steps_per_launch, blocks_per_cu, target, trials;
+	double steps_per_s;
+} nbl_tune_result;
+int nbl_driver_autotune(nbl_driver* d, uint32_t iters_per_trial, nbl_tune_result* out);
 
 /* gpu_driver::get_running_count / get_detected_count
  * (drivers/gpu/gpu_driver.inl:140-162).  Waits for the stream.  *queued =
@@ -290,6 +323,44 @@ typedef struct {
  * secondary electron made (se_made[i] = 1). */
 int nbl_driver_step_states(nbl_driver* d, nbl_step_state* states, int32_t* event,
 	int32_t* outcome, nbl_step_state* se, int32_t* se_made, uint32_t n);
+
+/* ------------------------------------------ derivatives (no counterpart) --
+ * The reference has no derivative of its outputs.  These entry points let a
+ * caller estimate d(yield)/d(material parameter) by the likelihood-ratio
+ * method: a run with scores enabled replays the same trajectories (same
+ * seed and stream ids give the same random draws) and attaches to every
+ * detected electron the FP64 score sum_steps d log p(step) / d theta over
+ * its own steps and its ancestors' steps up to its creation.  The mean over
+ * primaries of (yield indicator x score) is the derivative of the mean
+ * yield.  Parameters per material: kind 0 = elastic mean-free-path scale,
+ * 1 = inelastic mean-free-path scale, 2 = barrier (work function + Fermi
+ * energy, eV).  Scores cover the random choices (free-path race, boundary
+ * transmission / absorption); the barrier's deterministic effects (energy
+ * shift, refraction, threshold) are not part of them. */
+#define NBL_KIND_EL_MFP_SCALE 0
+#define NBL_KIND_IN_MFP_SCALE 1
+#define NBL_KIND_BARRIER 2
+#define NBL_MAX_SCORE_PARAMS 8
+
+/* Model parameters of material `material` for this driver: free paths are
+ * sampled with the mean free path x scale; `barrier` replaces the file's. */
+int nbl_driver_set_material_params(nbl_driver* d, int32_t material, float el_mfp_scale,
+	float in_mfp_scale, float barrier);
+int nbl_driver_get_material_params(nbl_driver* d, int32_t material, float* el_mfp_scale,
+	float* in_mfp_scale, float* barrier);
+
+/* Enable scores for n (<= 8) parameters (materials[i], kinds[i]), or
+ * disable with n = 0.  Only while nothing is in flight, queued or waiting
+ * to be drained (NBL_ERR_ARG otherwise); primaries start with score 0. */
+int nbl_driver_set_score_params(nbl_driver* d, const int32_t* materials, const int32_t* kinds,
+	int32_t n);
+
+/* nbl_driver_drain_detected / nbl_driver_run plus scores[i * n + k] for
+ * record i and parameter k. */
+int nbl_driver_drain_detected_scores(nbl_driver* d, nbl_detected* out, double* scores,
+	uint32_t cap, uint32_t* n);
+int nbl_driver_run_scores(nbl_driver* d, nbl_detected* out, double* scores, uint64_t cap,
+	uint64_t* n_detected);
 
 #ifdef __cplusplus
 }

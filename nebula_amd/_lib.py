@@ -26,6 +26,7 @@ NBL_OK, NBL_ERR_ARG, NBL_ERR_IO, NBL_ERR_FORMAT, NBL_ERR_ALLOC, NBL_ERR_DEVICE, 
     NBL_ERR_CAPACITY, NBL_ERR_UNSUPPORTED = 0, -1, -2, -3, -4, -5, -6, -7
 
 NBL_INEL_PENN, NBL_INEL_KIEFT = 0, 1
+KIND_EL_MFP_SCALE, KIND_IN_MFP_SCALE, KIND_BARRIER = 0, 1, 2
 NBL_PHYS_DEFAULT = 0x1FF & ~(1 << 9)
 
 
@@ -69,6 +70,11 @@ class RawMaterial(C.Structure):
                 ("pn_q", RawDataset), ("pn_ion", RawDataset), ("ki_ion", RawDataset),
                 ("outer_shells", C.POINTER(C.c_double)), ("n_outer_shells", C.c_int32),
                 ("outer_shells_unit", C.c_double)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("steps_per_launch", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("target", C.c_uint32),
+                ("trials", C.c_uint32), ("steps_per_s", C.c_double)]
 
 
 class Table(C.Structure):
@@ -132,6 +138,12 @@ _SIGS = {
     "nbl_driver_push": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64,
                                   _p(C.c_uint32)]),
     "nbl_driver_iterate": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "nbl_driver_set_launch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "nbl_driver_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "nbl_driver_autotune": (C.c_int, [C.c_void_p, C.c_uint32, _p(TuneResult)]),
+    "nbl_driver_add_world": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, _p(C.c_int32)]),
+    "nbl_driver_push_world": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64,
+                                        C.c_int32, _p(C.c_uint32)]),
     "nbl_driver_counts": (C.c_int, [C.c_void_p, _p(C.c_uint32), _p(C.c_uint32), _p(C.c_uint64)]),
     "nbl_driver_drain_detected": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, _p(C.c_uint32)]),
     "nbl_driver_steps": (C.c_int, [C.c_void_p, _p(C.c_uint64)]),
@@ -146,6 +158,13 @@ _SIGS = {
     "nbl_device_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, _p(C.c_float)]),
     "nbl_driver_step_states": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_uint32]),
+    "nbl_driver_set_material_params": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_float]),
+    "nbl_driver_get_material_params": (C.c_int, [C.c_void_p, C.c_int32, _p(C.c_float), _p(C.c_float),
+                                                 _p(C.c_float)]),
+    "nbl_driver_set_score_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
+    "nbl_driver_drain_detected_scores": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                                   _p(C.c_uint32)]),
+    "nbl_driver_run_scores": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, _p(C.c_uint64)]),
 }
 
 

@@ -13,7 +13,7 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import (DET_DTYPE, PART_DTYPE, STATE_DTYPE, TRI_DTYPE, MaterialHost, RawDataset,
+from ._lib import (DET_DTYPE, PART_DTYPE, STATE_DTYPE, TRI_DTYPE, MaterialHost, NebulaError, RawDataset,
                    RawMaterial, check, lib, ptr)
 from .workloads import Mesh, PRI_DTYPE
 from .workloads import RawMaterial as PyRawMaterial
@@ -384,6 +384,7 @@ class Driver:
                  physics: int = _lib.NBL_PHYS_DEFAULT, energy_threshold: float = 0.0,
                  seed: int = 0x78C7E39B, steps_per_launch: int = 4):
         self.device, self.geometry, self.materials = device, geometry, list(materials)
+        self.capacity = capacity
         arr = (C.c_void_p * max(1, len(self.materials)))(*[m._h for m in self.materials])
         self._h = C.c_void_p()
         check(lib().nbl_driver_create(device._h, capacity, geometry._h, arr, len(self.materials),
@@ -391,13 +392,36 @@ class Driver:
                                       C.byref(self._h)))
         self.seed = seed
 
-    def push(self, particles: np.ndarray, tags=None, sid_base: int = 0) -> int:
+    def push(self, particles: np.ndarray, tags=None, sid_base: int = 0, world: int = 0) -> int:
         p = np.ascontiguousarray(to_particles(particles) if particles.dtype != PART_DTYPE else particles)
         t = None if tags is None else np.ascontiguousarray(tags, np.uint32)
         pushed = C.c_uint32()
-        check(lib().nbl_driver_push(self._h, ptr(p), None if t is None else ptr(t), len(p), sid_base,
-                                    C.byref(pushed)))
+        check(lib().nbl_driver_push_world(self._h, ptr(p), None if t is None else ptr(t), len(p), sid_base,
+                                          world, C.byref(pushed)))
         return pushed.value
+
+    def set_launch(self, steps_per_launch: int, blocks_per_cu: int, target: int) -> None:
+        check(lib().nbl_driver_set_launch(self._h, steps_per_launch, blocks_per_cu, target))
+
+    def set_graph(self, enable: bool) -> None:
+        check(lib().nbl_driver_set_graph(self._h, int(bool(enable))))
+
+    def autotune(self, iters_per_trial: int = 4) -> dict:
+        """Measured search over the launch configuration on the queued work;
+        the best one stays set."""
+        r = _lib.TuneResult()
+        check(lib().nbl_driver_autotune(self._h, iters_per_trial, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in r._fields_}
+
+    def add_world(self, geometry: "Geometry", materials) -> int:
+        """Another geometry with its own materials tracked by this driver;
+        returns the world id for push(..., world=id)."""
+        mats = list(materials)
+        arr = (C.c_void_p * max(1, len(mats)))(*[m._h for m in mats])
+        w = C.c_int32()
+        check(lib().nbl_driver_add_world(self._h, geometry._h, arr, len(mats), C.byref(w)))
+        self._keep = getattr(self, "_keep", []) + [geometry] + mats
+        return w.value
 
     def do_iteration(self, n: int = 1) -> None:
         check(lib().nbl_driver_iterate(self._h, n))
@@ -426,6 +450,68 @@ class Driver:
         n = C.c_uint64()
         check(lib().nbl_driver_run(self._h, ptr(out), cap, C.byref(n)))
         return out[: n.value]
+
+    # -- derivatives (likelihood-ratio scores; no reference counterpart) --
+    def set_material_params(self, material: int, el_mfp_scale: float = None, in_mfp_scale: float = None,
+                            barrier: float = None) -> None:
+        """Mean-free-path scales and barrier (eV) of `material` for this
+        driver; unspecified values are kept."""
+        e, i, b = self.get_material_params(material)
+        check(lib().nbl_driver_set_material_params(
+            self._h, material, e if el_mfp_scale is None else el_mfp_scale,
+            i if in_mfp_scale is None else in_mfp_scale, b if barrier is None else barrier))
+
+    def get_material_params(self, material: int):
+        e, i, b = C.c_float(), C.c_float(), C.c_float()
+        check(lib().nbl_driver_get_material_params(self._h, material, C.byref(e), C.byref(i), C.byref(b)))
+        return e.value, i.value, b.value
+
+    def set_score_params(self, params) -> None:
+        """params: list of (material, kind), kind 0 / 1 / 2 = elastic MFP
+        scale / inelastic MFP scale / barrier; [] disables scores."""
+        m = np.array([p[0] for p in params], np.int32)
+        k = np.array([p[1] for p in params], np.int32)
+        check(lib().nbl_driver_set_score_params(self._h, ptr(m) if len(m) else None,
+                                                ptr(k) if len(k) else None, len(params)))
+        self.n_params = len(params)
+
+    def run_scores(self, cap: int):
+        """run() plus the (n_detected, n_params) FP64 scores."""
+        np_ = getattr(self, "n_params", 0)
+        if not np_:
+            raise NebulaError(_lib.NBL_ERR_ARG, "scores are not enabled (set_score_params)")
+        out = np.zeros(max(cap, 1), dtype=DET_DTYPE)
+        sc = np.zeros((max(cap, 1), np_), np.float64)
+        n = C.c_uint64()
+        check(lib().nbl_driver_run_scores(self._h, ptr(out), ptr(sc), cap, C.byref(n)))
+        return out[: n.value], sc[: n.value]
+
+    def run_tally(self, tag_bin: np.ndarray, n_bins: int, frame: int = 32, se_cutoff: float = 50.0):
+        """Track everything pushed to the end without keeping the records:
+        counts[bin, 0] = detected electrons below `se_cutoff` eV (SE) and
+        counts[bin, 1] = the others (BSE), binned by tag_bin[tag].  The
+        detected buffer is drained after every `frame` iterations."""
+        counts = np.zeros((n_bins, 2), np.int64)
+        buf = np.zeros(max(1, self.capacity), dtype=DET_DTYPE)
+        n = C.c_uint32()
+        last_steps, stalled = -1, 0
+        while True:
+            self.do_iteration(frame)
+            running, det, queued = self.counts()
+            if det:
+                check(lib().nbl_driver_drain_detected(self._h, ptr(buf), len(buf), C.byref(n)))
+                got = buf[: n.value]
+                b = tag_bin[got["tag"]]
+                se = got["E"] < se_cutoff
+                counts[:, 0] += np.bincount(b[se], minlength=n_bins)
+                counts[:, 1] += np.bincount(b[~se], minlength=n_bins)
+            if running == 0 and queued == 0 and det == 0:
+                return counts
+            st = self.steps()
+            stalled = stalled + 1 if (st == last_steps and det == 0) else 0
+            last_steps = st
+            if stalled >= 4:
+                raise NebulaError(_lib.NBL_ERR_CAPACITY, "no progress: raise the capacity")
 
     def steps(self) -> int:
         s = C.c_uint64()

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -34,6 +36,20 @@ struct Bufs {
 	int *mat, *last_tri;
 	uint32_t *tag, *step, *flags;
 	uint64_t* sid;
+	double* sc;             // scores, [parameter][entry], or null
+	uint32_t sc_stride;     // entries of this buffer
+};
+
+// Likelihood-ratio scores carried by a particle (FP64 accumulation).
+constexpr int MAX_PARAMS = 8;
+constexpr int MAX_SCORED_MATERIALS = 16;
+struct Score { double v[MAX_PARAMS]; };
+
+// Parameter index of (material, kind) or -1; kinds 0 elastic MFP scale,
+// 1 inelastic MFP scale, 2 barrier.
+struct ScoreMap {
+	int np;
+	int8_t pidx[MAX_SCORED_MATERIALS][3];
 };
 
 struct Ctrl {
@@ -48,6 +64,8 @@ struct Ctrl {
 	uint32_t spill_ctr;     // spill stack top during a k_step (may overshoot)
 	uint32_t spill_take, spill_src;  // refill from the spill stack
 	uint32_t inel_ctr;      // particles queued for their inelastic event
+	uint32_t work_ctr;      // next input particle for a k_step lane
+	uint32_t pad1;
 	uint64_t refill_src;
 	uint64_t queue_n, queue_next;
 	uint64_t steps;         // particle-steps (init calls on active particles)
@@ -55,11 +73,14 @@ struct Ctrl {
 
 struct StepArgs {
 	World W;
+	const WorldDesc* worlds;    // per-world geometry and materials (MULTI kernels)
 	Bufs buf[2];
 	Bufs spill;
 	Bufs inel;
 	Ctrl* ctrl;
 	nbl_detected* det;
+	double* det_sc;         // scores of detected records, [record][parameter]
+	ScoreMap map;
 	uint32_t det_cap;
 	uint32_t slots;
 	uint32_t spill_cap;
@@ -92,6 +113,62 @@ __device__ __forceinline__ void store_state(const Bufs& b, uint32_t i, const Sta
 	b.step[i] = s.step;
 	b.flags[i] = s.flags;
 	b.sid[i] = s.sid;
+}
+
+__device__ __forceinline__ void load_score(const Bufs& b, uint32_t i, int np, Score& sc)
+{
+#pragma unroll
+	for (int q = 0; q < MAX_PARAMS; ++q)
+		sc.v[q] = (q < np) ? b.sc[(size_t)q * b.sc_stride + i] : 0.0;
+}
+
+__device__ __forceinline__ void store_score(const Bufs& b, uint32_t i, int np, const Score& sc)
+{
+#pragma unroll
+	for (int q = 0; q < MAX_PARAMS; ++q)
+		if (q < np)
+			b.sc[(size_t)q * b.sc_stride + i] = sc.v[q];
+}
+
+// score[q] += inc for a runtime q, without indexing registers dynamically
+__device__ __forceinline__ void score_add(Score& sc, int q, float inc)
+{
+#pragma unroll
+	for (int j = 0; j < MAX_PARAMS; ++j)
+		if (j == q)
+			sc.v[j] += (double)inc;
+}
+
+__device__ __forceinline__ int score_index(const ScoreMap& m, int mat, int kind)
+{
+	return (mat >= 0 && mat < MAX_SCORED_MATERIALS) ? (int)m.pidx[mat][kind] : -1;
+}
+
+// d log p / d theta of a step's free-path race in material `mat`: rates
+// 1/tau, travelled distance d, winner `event` (or a boundary first).
+__device__ __forceinline__ void score_free_path(Score& sc, const ScoreMap& map, const World& W,
+	int mat, const PathInfo& pi, int event)
+{
+	if (pi.type == 0)
+		return;
+	const Material& m = W.mats[mat];
+	const int pe = score_index(map, mat, 0), pn = score_index(map, mat, 1);
+	if (pe >= 0)
+		score_add(sc, pe, (pi.dist / pi.tau_el - ((event == EV_ELASTIC) ? 1.0f : 0.0f)) / m.el_scale);
+	if (pn >= 0)
+		score_add(sc, pn, (pi.dist / pi.tau_in - ((event == EV_INELASTIC) ? 1.0f : 0.0f)) / m.in_scale);
+}
+
+// d log p / d b for the barriers on either side of a boundary (dU = b_mo - b_mi)
+__device__ __forceinline__ void score_barrier(Score& sc, const ScoreMap& map, int mi, int mo, float g)
+{
+	if (g == 0.0f)
+		return;
+	const int po = score_index(map, mo, 2), pi = score_index(map, mi, 2);
+	if (po >= 0)
+		score_add(sc, po, g);
+	if (pi >= 0)
+		score_add(sc, pi, -g);
 }
 
 // Slot allocation for the lanes of a wavefront with `want` set: one atomic
@@ -144,6 +221,7 @@ __global__ void k_ctrl(Ctrl* c, uint32_t slots, uint32_t spill_cap)
 	c->surv_ctr = 0;
 	c->se_ctr = 0;
 	c->inel_ctr = 0;
+	c->work_ctr = 0;
 	c->spill_ctr = c->spill_n;
 	c->stepped = 1;
 }
@@ -153,7 +231,7 @@ __global__ void k_ctrl(Ctrl* c, uint32_t slots, uint32_t spill_cap)
 // material, no last triangle).
 __global__ __launch_bounds__(256) void k_refill(Ctrl* c, Bufs b0, Bufs b1, Bufs spill,
 	const nbl_particle28* __restrict__ q, const uint32_t* __restrict__ qtag,
-	const uint64_t* __restrict__ qsid)
+	const uint64_t* __restrict__ qsid, const uint16_t* __restrict__ qworld, uint32_t np)
 {
 	const uint32_t n = c->refill_n;
 	const uint32_t stride = gridDim.x * blockDim.x;
@@ -163,6 +241,9 @@ __global__ __launch_bounds__(256) void k_refill(Ctrl* c, Bufs b0, Bufs b1, Bufs 
 	const uint32_t take = c->spill_take;
 	if (i < take) {
 		store_state(b, dst, load_state(spill, c->spill_src + i));
+		if (b.sc)
+			for (uint32_t q = 0; q < np; ++q)
+				b.sc[(size_t)q * b.sc_stride + dst] = spill.sc[(size_t)q * spill.sc_stride + c->spill_src + i];
 		continue;
 	}
 	const uint64_t src = c->refill_src + (i - take);
@@ -175,9 +256,12 @@ __global__ __launch_bounds__(256) void k_refill(Ctrl* c, Bufs b0, Bufs b1, Bufs 
 	s.last_tri = -1;
 	s.tag = qtag[src];
 	s.step = 0;
-	s.flags = 0;
+	s.flags = qworld ? ((uint32_t)qworld[src] << FL_WORLD_SHIFT) : 0u;
 	s.sid = qsid[src];
 	store_state(b, dst, s);
+	if (b.sc)
+		for (uint32_t q = 0; q < np; ++q)
+			b.sc[(size_t)q * b.sc_stride + dst] = 0.0;
 	}
 }
 
@@ -192,9 +276,11 @@ __device__ __forceinline__ void count_steps(Ctrl* C, uint32_t my_steps)
 }
 
 // Place a new secondary: the output buffer's SE region, else the spill
-// stack; returns false when both are full.
+// stack; returns false when both are full.  With scores, the secondary
+// starts from its parent's score.
+template <bool SC>
 __device__ __forceinline__ bool place_secondary(const StepArgs& a, Ctrl* C, const Bufs& out,
-	uint32_t se_limit, bool made, const Particle& se, const State& parent)
+	uint32_t se_limit, bool made, const Particle& se, const State& parent, const Score& psc)
 {
 	const uint32_t sslot = wave_alloc(&C->se_ctr, made);
 	bool ok = true;
@@ -205,16 +291,19 @@ __device__ __forceinline__ bool place_secondary(const StepArgs& a, Ctrl* C, cons
 		ns.last_tri = -1;
 		ns.tag = parent.tag;
 		ns.step = 0;
-		ns.flags = 0;
+		ns.flags = parent.flags & FL_WORLD_MASK;
 		ns.sid = se_sid(parent.sid, parent.step);
 		const bool spill = sslot >= se_limit;
 		const uint32_t pslot = wave_alloc(&C->spill_ctr, spill);
-		if (!spill)
+		if (!spill) {
 			store_state(out, a.slots - 1u - sslot, ns);
-		else if (pslot < a.spill_cap)
+			if (SC) store_score(out, a.slots - 1u - sslot, a.map.np, psc);
+		} else if (pslot < a.spill_cap) {
 			store_state(a.spill, pslot, ns);
-		else
+			if (SC) store_score(a.spill, pslot, a.map.np, psc);
+		} else {
 			ok = false;
+		}
 	}
 	return ok;
 }
@@ -224,6 +313,18 @@ __device__ __forceinline__ bool place_secondary(const StepArgs& a, Ctrl* C, cons
 // inelastic stops and joins the inelastic queue, so that the long inelastic
 // code runs in k_inel on waves where every lane needs it (about 7% of the
 // steps of a 300 eV scan would otherwise put it into nearly every wave).
+//
+// Lanes are persistent: a lane takes an input particle (wave-wide ballot,
+// one atomic per wave), steps it until it leaves (detected, terminated,
+// inelastic event) or has taken K steps, writes it out and takes the next
+// one.  A particle that leaves early therefore frees its lane at once
+// instead of idling it for the rest of the K steps.
+//
+// SC: also accumulate each particle's likelihood-ratio score (free-path
+// race and boundary choices) and write it with the particle.
+// MULTI: particles belong to different worlds (flags bits 16..31); each lane
+// takes its particle's geometry and materials from the world table.
+template <bool SC, bool MULTI>
 __global__ __launch_bounds__(256) void k_step(StepArgs a)
 {
 	Ctrl* C = a.ctrl;
@@ -231,72 +332,122 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
 	const Bufs in = bi ? a.buf[1] : a.buf[0];
 	const Bufs out = bi ? a.buf[0] : a.buf[1];
 	const uint32_t n_surv = C->n_surv, n_in = n_surv + C->n_se;
-	const uint32_t B = a.slots, det_cap = a.det_cap;
-	const World& W = a.W;
+	const uint32_t B = a.slots, det_cap = a.det_cap, K = a.K;
+	const int np = a.map.np;
+	World Wl = a.W;
+	const World& W = MULTI ? Wl : a.W;
 	__shared__ int s_path[PATH_LEVELS * 256];
 	int* path = s_path + threadIdx.x;
 	uint32_t my_steps = 0;
-	const uint32_t stride = gridDim.x * blockDim.x;
-	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_in; i += stride) {
-		const uint32_t slot = (i < n_surv) ? i : (B - 1u - (i - n_surv));
-		State s = load_state(in, slot);
-		bool keep = true, to_inel = false;
-		for (uint32_t k = 0; k < a.K; ++k) {
-			int oc = ALIVE;
-			if (s.flags & FL_PEND_DET) {
-				oc = DETECTED;
-			} else if (s.flags & FL_PEND_INEL) {
-				// an inelastic event that found no secondary slot earlier
-				s.flags &= ~FL_PEND_INEL;
-				to_inel = true;
+	State s;
+	Score sc;
+	bool have = false, drained = false;
+	uint32_t k = 0;
+	for (;;) {
+		const bool want = !have && !drained;
+		const uint32_t i = wave_alloc(&C->work_ctr, want);
+		if (want) {
+			if (i < n_in) {
+				const uint32_t slot = (i < n_surv) ? i : (B - 1u - (i - n_surv));
+				s = load_state(in, slot);
+				if (SC) load_score(in, slot, np, sc);
+				if (MULTI) {
+					const WorldDesc& wd = a.worlds[s.flags >> FL_WORLD_SHIFT];
+					Wl.g = wd.g;
+					Wl.mats = wd.mats;
+					Wl.nmats = wd.nmats;
+				}
+				have = true;
+				k = 0;
 			} else {
-				int ev = EV_NONE, hit = -1;
-				Rng rng;
-				rng_start(rng, W.key0, W.key1, s.sid, s.step);
-				oc = step_init(W, s, rng, ev, hit, path, 256);
-				++my_steps;
-				if (oc == ALIVE) {
-					if (ev == EV_INELASTIC) {
-						to_inel = true;
+				drained = true;
+			}
+		}
+		if (!__any(have))
+			break;
+		if (!have)
+			continue;
+		int oc = ALIVE;
+		bool to_inel = false;
+		if (s.flags & FL_PEND_DET) {
+			oc = DETECTED;
+		} else if (s.flags & FL_PEND_INEL) {
+			// an inelastic event that found no secondary slot earlier
+			s.flags &= ~FL_PEND_INEL;
+			to_inel = true;
+		} else {
+			int ev = EV_NONE, hit = -1;
+			Rng rng;
+			rng_start(rng, W.key0, W.key1, s.sid, s.step);
+			const int mat0 = s.mat;
+			PathInfo pinfo;
+			oc = step_init(W, s, rng, ev, hit, path, 256, pinfo);
+			++my_steps;
+			if (SC && oc == ALIVE)
+				score_free_path(sc, a.map, W, mat0, pinfo, ev);
+			if (oc == ALIVE) {
+				if (ev == EV_INELASTIC) {
+					to_inel = true;
+				} else {
+					Particle unused;
+					bool made = false;
+					if (SC) {
+						float g = 0.0f;
+						int gmi = -1, gmo = -1;
+						oc = step_event(W, s, rng, ev, hit, unused, made, &g, &gmi, &gmo);
+						if (ev == EV_INTERSECT)
+							score_barrier(sc, a.map, gmi, gmo, g);
 					} else {
-						Particle unused;
-						bool made = false;
 						oc = step_event(W, s, rng, ev, hit, unused, made);
 					}
 				}
 			}
-			bool stop = false;
-			const bool det = (oc == DETECTED);
-			const uint32_t dslot = wave_alloc(&C->det_ctr, det);
-			if (det) {
-				if (dslot < det_cap) {
-					nbl_detected r;
-					r.pos[0] = s.p.pos.x; r.pos[1] = s.p.pos.y; r.pos[2] = s.p.pos.z;
-					r.dir[0] = s.p.dir.x; r.dir[1] = s.p.dir.y; r.dir[2] = s.p.dir.z;
-					r.kin_energy = s.p.E;
-					r.tag = s.tag;
-					r.sid = s.sid;
-					a.det[dslot] = r;
-					keep = false;
-				} else {
-					s.flags |= FL_PEND_DET;
-					stop = true;
-				}
-			}
-			if (oc == TERMINATED)
-				keep = false;
-			if (!keep || stop || to_inel)
-				break;
-			s.step++;
 		}
-		const bool queue = keep && to_inel;
+		bool keep = true, stop = false;
+		const bool det = (oc == DETECTED);
+		const uint32_t dslot = wave_alloc(&C->det_ctr, det);
+		if (det) {
+			if (dslot < det_cap) {
+				nbl_detected r;
+				r.pos[0] = s.p.pos.x; r.pos[1] = s.p.pos.y; r.pos[2] = s.p.pos.z;
+				r.dir[0] = s.p.dir.x; r.dir[1] = s.p.dir.y; r.dir[2] = s.p.dir.z;
+				r.kin_energy = s.p.E;
+				r.tag = s.tag;
+				r.sid = s.sid;
+				a.det[dslot] = r;
+				if (SC) {
+#pragma unroll
+					for (int q = 0; q < MAX_PARAMS; ++q)
+						if (q < np)
+							a.det_sc[(size_t)dslot * np + q] = sc.v[q];
+				}
+				keep = false;
+			} else {
+				s.flags |= FL_PEND_DET;
+				stop = true;
+			}
+		}
+		if (oc == TERMINATED)
+			keep = false;
+		if (keep && !stop && !to_inel) {
+			s.step++;
+			++k;
+		}
+		const bool fin = !keep || stop || to_inel || k == K;
+		const bool queue = fin && keep && to_inel;
 		const uint32_t qslot = wave_alloc(&C->inel_ctr, queue);
-		if (queue)
+		if (queue) {
 			store_state(a.inel, qslot, s);
-		const bool survive = keep && !to_inel;
+			if (SC) store_score(a.inel, qslot, np, sc);
+		}
+		const bool survive = fin && keep && !to_inel;
 		const uint32_t oslot = wave_alloc(&C->surv_ctr, survive);
-		if (survive)
+		if (survive) {
 			store_state(out, oslot, s);
+			if (SC) store_score(out, oslot, np, sc);
+		}
+		if (fin)
+			have = false;
 	}
 	count_steps(C, my_steps);
 }
@@ -304,16 +455,27 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
 // The inelastic events queued by k_step (drivers/gpu/gpu_driver.inl:505-540):
 // forget the last triangle, run the inelastic model from draw 2 of the step's
 // stream, place the secondary, and keep the particle for the next launch.
+// The inelastic models carry no parameter of the score.
+template <bool SC, bool MULTI>
 __global__ __launch_bounds__(256) void k_inel(StepArgs a)
 {
 	Ctrl* C = a.ctrl;
 	const uint32_t bi = C->buf_in;
 	const Bufs out = bi ? a.buf[0] : a.buf[1];
 	const uint32_t n = C->inel_ctr, se_limit = C->se_limit;
-	const World& W = a.W;
+	const int np = a.map.np;
+	World Wl = a.W;
+	const World& W = MULTI ? Wl : a.W;
 	const uint32_t stride = gridDim.x * blockDim.x;
 	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
 		State s = load_state(a.inel, i);
+		if (MULTI) {
+			const WorldDesc& wd = a.worlds[s.flags >> FL_WORLD_SHIFT];
+			Wl.mats = wd.mats;
+			Wl.nmats = wd.nmats;
+		}
+		Score sc;
+		if (SC) load_score(a.inel, i, np, sc);
 		const State pre = s;
 		Rng rng;
 		rng_start(rng, W.key0, W.key1, s.sid, s.step);
@@ -321,7 +483,7 @@ __global__ __launch_bounds__(256) void k_inel(StepArgs a)
 		Particle se;
 		bool made = false;
 		step_event(W, s, rng, EV_INELASTIC, -1, se, made);
-		if (place_secondary(a, C, out, se_limit, made, se, s)) {
+		if (place_secondary<SC>(a, C, out, se_limit, made, se, s, sc)) {
 			s.step++;
 		} else {
 			// no room anywhere: retry the event from the same draws later
@@ -330,6 +492,7 @@ __global__ __launch_bounds__(256) void k_inel(StepArgs a)
 		}
 		const uint32_t oslot = wave_alloc(&C->surv_ctr, true);
 		store_state(out, oslot, s);
+		if (SC) store_score(out, oslot, np, sc);
 	}
 }
 
@@ -406,7 +569,8 @@ __global__ __launch_bounds__(256) void k_step_states(World W, nbl_step_state* st
 	Rng rng;
 	rng_start(rng, W.key0, W.key1, s.sid, s.step);
 	int ev = EV_NONE, hit = -1;
-	int oc = step_init(W, s, rng, ev, hit, s_path + threadIdx.x, 256);
+	PathInfo pinfo;
+	int oc = step_init(W, s, rng, ev, hit, s_path + threadIdx.x, 256, pinfo);
 	Particle se;
 	bool made = false;
 	if (oc == ALIVE)
@@ -433,6 +597,7 @@ __global__ __launch_bounds__(256) void k_step_states(World W, nbl_step_state* st
 // ------------------------------------------------------- host objects --
 struct nbl_device {
 	int ordinal;
+	int n_cu;           // compute units, for persistent grids
 	hipStream_t stream;
 };
 
@@ -469,8 +634,22 @@ struct nbl_driver {
 	nbl_particle28* q_part;
 	uint32_t* q_tag;
 	uint64_t* q_sid;
+	uint16_t* q_world;
 	uint64_t q_cap, q_n;
+	std::vector<WorldDesc> h_worlds;   // world 0 = the creation geometry
+	std::vector<Material*> world_mats; // device material arrays of worlds 1..
+	WorldDesc* d_worlds;
 	uint32_t next_tag;
+	ScoreMap map;           // map.np == 0: no scores
+	double* sc_mem[4];      // score arrays of buf[0], buf[1], spill, inel
+	double* det_sc;
+	std::vector<Material> h_mats;   // host copy of d_mats
+	uint32_t blocks_per_cu = 8;     // persistent k_step grid
+	uint32_t target = 0;            // particles in flight to aim for
+	bool use_graph = false;         // replay iterations from captured graphs
+	uint64_t epoch = 0;             // bumped when a launch argument changes
+	struct Graph { uint32_t n; uint64_t epoch; hipGraph_t g; hipGraphExec_t exec; };
+	std::vector<Graph> graphs;
 	bool timing;
 	std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
 	double step_ms;
@@ -542,6 +721,8 @@ int alloc_bufs(Bufs& b, void*& mem, uint32_t slots)
 	b.tag = (uint32_t*)p; p += 4 * n;
 	b.step = (uint32_t*)p; p += 4 * n;
 	b.flags = (uint32_t*)p; p += 4 * n;
+	b.sc = nullptr;
+	b.sc_stride = slots;
 	return NBL_OK;
 }
 
@@ -588,8 +769,11 @@ int nbl_device_open(int ordinal, nbl_device** out)
 	if (ordinal < 0 || ordinal >= n)
 		return fail(NBL_ERR_ARG, "no such GPU ordinal " + std::to_string(ordinal));
 	HIP_OK(hipSetDevice(ordinal));
+	int n_cu = 0;
+	HIP_OK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ordinal));
 	auto* d = new nbl_device;
 	d->ordinal = ordinal;
+	d->n_cu = n_cu > 0 ? n_cu : 1;
 	hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
 	if (e != hipSuccess) {
 		delete d;
@@ -744,6 +928,8 @@ int nbl_material_upload(nbl_device* dev, const nbl_material_host* m, nbl_materia
 	d.recoil_const = m->recoil_const;
 	d.fermi = m->fermi;
 	d.band_gap = m->band_gap;
+	d.el_scale = 1.0f;
+	d.in_scale = 1.0f;
 	d.inel_model = m->inel_model;
 	d.el_imfp = tab1(m->el_imfp, mm->tables[0]);
 	d.el_icdf = tab2(m->el_icdf, mm->tables[1]);
@@ -792,6 +978,14 @@ int nbl_driver_destroy(nbl_driver* d)
 	hipFree(d->q_part);
 	hipFree(d->q_tag);
 	hipFree(d->q_sid);
+	hipFree(d->q_world);
+	for (auto& gr : d->graphs) { hipGraphExecDestroy(gr.exec); hipGraphDestroy(gr.g); }
+	for (Material* p : d->world_mats)
+		hipFree(p);
+	hipFree(d->d_worlds);
+	for (double* p : d->sc_mem)
+		hipFree(p);
+	hipFree(d->det_sc);
 	hipStreamDestroy(d->stream);
 	delete d;
 	return NBL_OK;
@@ -844,9 +1038,13 @@ int nbl_driver_create(nbl_device* dev, uint32_t capacity, const nbl_geometry* ge
 		hm[i] = mats[i]->m;
 	if (hipMemcpy(d->d_mats, hm.data(), sizeof(Material) * hm.size(), hipMemcpyHostToDevice) != hipSuccess)
 		return cleanup(fail(NBL_ERR_DEVICE, "material descriptor upload failed"));
+	d->h_mats = hm;
+	d->map.np = 0;
+	std::memset(d->map.pidx, -1, sizeof(d->map.pidx));
 	Ctrl c;
 	std::memset(&c, 0, sizeof(c));
 	c.target = capacity;
+	d->target = capacity;
 	if (hipMemcpy(d->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice) != hipSuccess)
 		return cleanup(fail(NBL_ERR_DEVICE, "control block upload failed"));
 	d->W.g = geom->g;
@@ -856,32 +1054,89 @@ int nbl_driver_create(nbl_device* dev, uint32_t capacity, const nbl_geometry* ge
 	d->W.threshold = energy_threshold;
 	d->W.key0 = (uint32_t)seed;
 	d->W.key1 = (uint32_t)(seed >> 32);
+	WorldDesc w0;
+	w0.g = geom->g;
+	w0.mats = d->d_mats;
+	w0.nmats = n_mats;
+	d->h_worlds.push_back(w0);
 	*out = d;
 	return NBL_OK;
 }
 
-int nbl_driver_push(nbl_driver* d, const nbl_particle28* particles, const uint32_t* tags,
-	uint32_t n, uint64_t sid_base, uint32_t* pushed)
+int nbl_driver_add_world(nbl_driver* d, const nbl_geometry* geom, const nbl_material* const* mats,
+	int32_t n_mats, int32_t* world)
+{
+	clear_error();
+	if (!d || !geom || !world || (n_mats > 0 && !mats) || n_mats < 0)
+		return fail(NBL_ERR_ARG, "invalid world arguments");
+	if (d->h_worlds.size() >= 65536)
+		return fail(NBL_ERR_CAPACITY, "at most 65536 worlds per driver");
+	if (geom->dev != d->dev)
+		return fail(NBL_ERR_ARG, "geometry on another device");
+	for (int i = 0; i < n_mats; ++i)
+		if (!mats[i] || mats[i]->dev != d->dev)
+			return fail(NBL_ERR_ARG, "material " + std::to_string(i) + " missing or on another device");
+	if (geom->max_material >= n_mats)
+		return fail(NBL_ERR_ARG, "not enough materials provided for this geometry: expected " +
+			std::to_string(geom->max_material + 1) + ", got " + std::to_string(n_mats));
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	Material* dm = nullptr;
+	if ((rc = dev_alloc(&dm, (size_t)std::max(1, n_mats))))
+		return rc;
+	std::vector<Material> hm((size_t)std::max(1, n_mats));
+	for (int i = 0; i < n_mats; ++i)
+		hm[i] = mats[i]->m;
+	if (hipMemcpy(dm, hm.data(), sizeof(Material) * hm.size(), hipMemcpyHostToDevice) != hipSuccess) {
+		hipFree(dm);
+		return fail(NBL_ERR_DEVICE, "material descriptor upload failed");
+	}
+	d->world_mats.push_back(dm);
+	WorldDesc w;
+	w.g = geom->g;
+	w.mats = dm;
+	w.nmats = n_mats;
+	d->h_worlds.push_back(w);
+	// the table is read by kernels in flight: replace it only when idle
+	HIP_OK(hipStreamSynchronize(d->stream));
+	WorldDesc* nw = nullptr;
+	if ((rc = dev_alloc(&nw, d->h_worlds.size())))
+		return rc;
+	HIP_OK(hipMemcpy(nw, d->h_worlds.data(), sizeof(WorldDesc) * d->h_worlds.size(), hipMemcpyHostToDevice));
+	hipFree(d->d_worlds);
+	d->d_worlds = nw;
+	d->epoch++;
+	*world = (int32_t)d->h_worlds.size() - 1;
+	return NBL_OK;
+}
+
+int nbl_driver_push_world(nbl_driver* d, const nbl_particle28* particles, const uint32_t* tags,
+	uint32_t n, uint64_t sid_base, int32_t world, uint32_t* pushed)
 {
 	clear_error();
 	if (!d || (n && !particles))
 		return fail(NBL_ERR_ARG, "invalid push arguments");
+	if (world < 0 || world >= (int32_t)d->h_worlds.size())
+		return fail(NBL_ERR_ARG, "no such world " + std::to_string(world));
 	int rc = set_device(d->dev);
 	if (rc) return rc;
 	HIP_OK(hipStreamSynchronize(d->stream));
 	// device queue: grow by doubling, keeping the unread part
 	if (d->q_n + n > d->q_cap) {
 		uint64_t nc = std::max<uint64_t>(d->q_cap ? 2 * d->q_cap : 1024, d->q_n + n);
-		nbl_particle28* np; uint32_t* nt; uint64_t* ns;
-		if ((rc = dev_alloc(&np, nc)) || (rc = dev_alloc(&nt, nc)) || (rc = dev_alloc(&ns, nc)))
+		nbl_particle28* np; uint32_t* nt; uint64_t* ns; uint16_t* nw;
+		if ((rc = dev_alloc(&np, nc)) || (rc = dev_alloc(&nt, nc)) || (rc = dev_alloc(&ns, nc)) ||
+			(rc = dev_alloc(&nw, nc)))
 			return rc;
 		if (d->q_n) {
 			HIP_OK(hipMemcpy(np, d->q_part, sizeof(nbl_particle28) * d->q_n, hipMemcpyDeviceToDevice));
 			HIP_OK(hipMemcpy(nt, d->q_tag, sizeof(uint32_t) * d->q_n, hipMemcpyDeviceToDevice));
 			HIP_OK(hipMemcpy(ns, d->q_sid, sizeof(uint64_t) * d->q_n, hipMemcpyDeviceToDevice));
+			HIP_OK(hipMemcpy(nw, d->q_world, sizeof(uint16_t) * d->q_n, hipMemcpyDeviceToDevice));
 		}
-		hipFree(d->q_part); hipFree(d->q_tag); hipFree(d->q_sid);
-		d->q_part = np; d->q_tag = nt; d->q_sid = ns;
+		hipFree(d->q_part); hipFree(d->q_tag); hipFree(d->q_sid); hipFree(d->q_world);
+		d->q_part = np; d->q_tag = nt; d->q_sid = ns; d->q_world = nw;
+		d->epoch++;
 		d->q_cap = nc;
 	}
 	std::vector<uint32_t> t(n);
@@ -894,6 +1149,8 @@ int nbl_driver_push(nbl_driver* d, const nbl_particle28* particles, const uint32
 	HIP_OK(hipMemcpy(d->q_part + d->q_n, particles, sizeof(nbl_particle28) * n, hipMemcpyHostToDevice));
 	HIP_OK(hipMemcpy(d->q_tag + d->q_n, t.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
 	HIP_OK(hipMemcpy(d->q_sid + d->q_n, s.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
+	const std::vector<uint16_t> wv(n, (uint16_t)world);
+	HIP_OK(hipMemcpy(d->q_world + d->q_n, wv.data(), sizeof(uint16_t) * n, hipMemcpyHostToDevice));
 	d->q_n += n;
 	const uint64_t qn = d->q_n;
 	HIP_OK(hipMemcpy(&d->ctrl->queue_n, &qn, sizeof(uint64_t), hipMemcpyHostToDevice));
@@ -902,13 +1159,37 @@ int nbl_driver_push(nbl_driver* d, const nbl_particle28* particles, const uint32
 	return NBL_OK;
 }
 
-int nbl_driver_iterate(nbl_driver* d, uint32_t n_iter)
+int nbl_driver_push(nbl_driver* d, const nbl_particle28* particles, const uint32_t* tags,
+	uint32_t n, uint64_t sid_base, uint32_t* pushed)
 {
-	clear_error();
-	if (!d)
-		return fail(NBL_ERR_ARG, "null driver");
-	int rc = set_device(d->dev);
-	if (rc) return rc;
+	return nbl_driver_push_world(d, particles, tags, n, sid_base, 0, pushed);
+}
+
+namespace {
+// k_step + k_inel in the variant the driver needs (scores, several worlds).
+void launch_step_pair(nbl_driver* d, const StepArgs& a, uint32_t g)
+{
+	const bool multi = d->h_worlds.size() > 1;
+	const dim3 grid(g), block(256);
+	if (d->map.np && multi) {
+		hipLaunchKernelGGL((k_step<true, true>), grid, block, 0, d->stream, a);
+		hipLaunchKernelGGL((k_inel<true, true>), grid, block, 0, d->stream, a);
+	} else if (d->map.np) {
+		hipLaunchKernelGGL((k_step<true, false>), grid, block, 0, d->stream, a);
+		hipLaunchKernelGGL((k_inel<true, false>), grid, block, 0, d->stream, a);
+	} else if (multi) {
+		hipLaunchKernelGGL((k_step<false, true>), grid, block, 0, d->stream, a);
+		hipLaunchKernelGGL((k_inel<false, true>), grid, block, 0, d->stream, a);
+	} else {
+		hipLaunchKernelGGL((k_step<false, false>), grid, block, 0, d->stream, a);
+		hipLaunchKernelGGL((k_inel<false, false>), grid, block, 0, d->stream, a);
+	}
+}
+} // namespace
+
+namespace {
+StepArgs step_args(const nbl_driver* d)
+{
 	StepArgs a;
 	a.W = d->W;
 	a.buf[0] = d->buf[0];
@@ -917,36 +1198,210 @@ int nbl_driver_iterate(nbl_driver* d, uint32_t n_iter)
 	a.spill_cap = d->spill_cap;
 	a.inel = d->inel;
 	a.ctrl = d->ctrl;
+	a.worlds = d->d_worlds;
 	a.det = d->det;
+	a.det_sc = d->det_sc;
+	a.map = d->map;
 	a.det_cap = d->det_cap;
 	a.slots = d->slots;
 	a.K = d->K;
+	return a;
+}
+
+// One tracking iteration: control, refill, step pair.  The step pair is
+// bracketed by timing events when timing is on.
+int enqueue_iteration(nbl_driver* d, const StepArgs& a)
+{
 	const uint32_t g_refill = blocks_for(d->capacity);
-	const uint32_t g_step = blocks_for(d->capacity);
-	for (uint32_t it = 0; it < n_iter; ++it) {
-		hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(64), 0, d->stream, d->ctrl, d->slots, d->spill_cap);
-		hipLaunchKernelGGL(k_refill, dim3(g_refill), dim3(256), 0, d->stream, d->ctrl, d->buf[0],
-			d->buf[1], d->spill, d->q_part, d->q_tag, d->q_sid);
-		std::pair<hipEvent_t, hipEvent_t> ev{};
-		if (d->timing) {
-			if (d->ev_free.empty()) {
-				HIP_OK(hipEventCreate(&ev.first));
-				HIP_OK(hipEventCreate(&ev.second));
-			} else {
-				ev = d->ev_free.back();
-				d->ev_free.pop_back();
-			}
-			HIP_OK(hipEventRecord(ev.first, d->stream));
+	// persistent lanes: blocks_per_cu blocks of 256 per compute unit, never
+	// more than the particles in flight need
+	const uint32_t g_step = std::min<uint32_t>(blocks_for(d->capacity), d->blocks_per_cu * (uint32_t)d->dev->n_cu);
+	hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(64), 0, d->stream, d->ctrl, d->slots, d->spill_cap);
+	hipLaunchKernelGGL(k_refill, dim3(g_refill), dim3(256), 0, d->stream, d->ctrl, d->buf[0],
+		d->buf[1], d->spill, d->q_part, d->q_tag, d->q_sid, d->q_world, (uint32_t)d->map.np);
+	std::pair<hipEvent_t, hipEvent_t> ev{};
+	if (d->timing) {
+		if (d->ev_free.empty()) {
+			HIP_OK(hipEventCreate(&ev.first));
+			HIP_OK(hipEventCreate(&ev.second));
+		} else {
+			ev = d->ev_free.back();
+			d->ev_free.pop_back();
 		}
-		hipLaunchKernelGGL(k_step, dim3(g_step), dim3(256), 0, d->stream, a);
-		hipLaunchKernelGGL(k_inel, dim3(g_step), dim3(256), 0, d->stream, a);
-		if (d->timing) {
-			HIP_OK(hipEventRecord(ev.second, d->stream));
-			d->ev_used.push_back(ev);
-			d->step_launches++;
+		HIP_OK(hipEventRecord(ev.first, d->stream));
+	}
+	launch_step_pair(d, a, g_step);
+	if (d->timing) {
+		HIP_OK(hipEventRecord(ev.second, d->stream));
+		d->ev_used.push_back(ev);
+		d->step_launches++;
+	}
+	return NBL_OK;
+}
+
+// n iterations as one captured graph (built on first use for this n and
+// this set of launch arguments).  The iterations contain kernels only.
+int launch_graph(nbl_driver* d, const StepArgs& a, uint32_t n)
+{
+	for (auto& gr : d->graphs)
+		if (gr.n == n && gr.epoch == d->epoch) {
+			HIP_OK(hipGraphLaunch(gr.exec, d->stream));
+			return NBL_OK;
+		}
+	// drop graphs made with stale arguments
+	for (auto it = d->graphs.begin(); it != d->graphs.end();) {
+		if (it->epoch != d->epoch) {
+			hipGraphExecDestroy(it->exec);
+			hipGraphDestroy(it->g);
+			it = d->graphs.erase(it);
+		} else {
+			++it;
 		}
 	}
+	nbl_driver::Graph gr{ n, d->epoch, nullptr, nullptr };
+	HIP_OK(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+	int rc = NBL_OK;
+	for (uint32_t it = 0; it < n && rc == NBL_OK; ++it)
+		rc = enqueue_iteration(d, a);
+	const hipError_t e = hipStreamEndCapture(d->stream, &gr.g);
+	if (rc) return rc;
+	if (e != hipSuccess)
+		return fail(NBL_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+	HIP_OK(hipGraphInstantiate(&gr.exec, gr.g, nullptr, nullptr, 0));
+	d->graphs.push_back(gr);
+	HIP_OK(hipGraphLaunch(gr.exec, d->stream));
+	return NBL_OK;
+}
+} // namespace
+
+int nbl_driver_iterate(nbl_driver* d, uint32_t n_iter)
+{
+	clear_error();
+	if (!d)
+		return fail(NBL_ERR_ARG, "null driver");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	const StepArgs a = step_args(d);
+	if (d->use_graph && !d->timing && n_iter > 1) {
+		if ((rc = launch_graph(d, a, n_iter)))
+			return rc;
+	} else {
+		for (uint32_t it = 0; it < n_iter; ++it)
+			if ((rc = enqueue_iteration(d, a)))
+				return rc;
+	}
 	HIP_OK(hipGetLastError());
+	return NBL_OK;
+}
+
+int nbl_driver_autotune(nbl_driver* d, uint32_t iters_per_trial, nbl_tune_result* out)
+{
+	clear_error();
+	if (!d || !out || iters_per_trial < 1)
+		return fail(NBL_ERR_ARG, "invalid autotune arguments");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	if ((rc = read_ctrl(d)))
+		return rc;
+	if (d->h_ctrl->queue_n == d->h_ctrl->queue_next)
+		return fail(NBL_ERR_ARG, "autotune measures queued primaries: push the work first");
+	const bool graph = d->use_graph;
+	d->use_graph = false;
+	struct Cfg { uint32_t K, bpc, target; };
+	Cfg best{ d->K, d->blocks_per_cu, d->target };
+	double best_rate = -1.0;
+	uint32_t trials = 0;
+	bool stop = false;
+	// particle-steps per second of `iters_per_trial` iterations on the queued work
+	auto trial = [&](const Cfg& c, double& rate) -> int {
+		int r = nbl_driver_set_launch(d, c.K, c.bpc, c.target);
+		if (r) return r;
+		if ((r = read_ctrl(d))) return r;
+		const uint64_t s0 = d->h_ctrl->steps;
+		const auto t0 = std::chrono::steady_clock::now();
+		if ((r = nbl_driver_iterate(d, iters_per_trial))) return r;
+		if ((r = read_ctrl(d))) return r;
+		const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+		rate = dt > 0 ? (double)(d->h_ctrl->steps - s0) / dt : 0.0;
+		++trials;
+		// stop before the detected buffer or the queue runs out
+		const uint64_t left = d->h_ctrl->queue_n - d->h_ctrl->queue_next;
+		if (d->h_ctrl->det_ctr > d->det_cap / 2 || left == 0)
+			stop = true;
+		return NBL_OK;
+	};
+	// ramp the population up with the current configuration
+	double rate = 0;
+	if ((rc = trial(best, rate))) { d->use_graph = graph; return rc; }
+	if ((rc = trial(best, rate))) { d->use_graph = graph; return rc; }
+	best_rate = rate;
+	const uint32_t Ks[] = { 4, 8, 16, 32 };
+	const uint32_t Bs[] = { 4, 5, 6, 8 };
+	const uint32_t Ts[] = { std::max<uint32_t>(64, d->capacity / 4), std::max<uint32_t>(64, d->capacity / 2),
+		d->capacity };
+	// coordinate search: steps per launch, then grid, then particles in flight
+	for (int dim = 0; dim < 3 && !stop; ++dim) {
+		const int nc = dim == 2 ? 3 : 4;
+		Cfg dim_best = best;
+		double dim_rate = best_rate;
+		for (int i = 0; i < nc && !stop; ++i) {
+			Cfg c = best;
+			if (dim == 0) c.K = Ks[i]; else if (dim == 1) c.bpc = Bs[i]; else c.target = Ts[i];
+			if (c.K == best.K && c.bpc == best.bpc && c.target == best.target)
+				continue;
+			if ((rc = trial(c, rate))) { d->use_graph = graph; return rc; }
+			if (rate > dim_rate) { dim_rate = rate; dim_best = c; }
+		}
+		// re-measure the incumbent: the work mix drifts while tuning
+		if (!stop && !(dim_best.K == best.K && dim_best.bpc == best.bpc && dim_best.target == best.target)) {
+			double again = 0;
+			if ((rc = trial(best, again))) { d->use_graph = graph; return rc; }
+			if (again >= dim_rate) dim_best = best, dim_rate = again;
+		}
+		best = dim_best;
+		best_rate = dim_rate;
+	}
+	d->use_graph = graph;
+	if ((rc = nbl_driver_set_launch(d, best.K, best.bpc, best.target)))
+		return rc;
+	out->steps_per_launch = best.K;
+	out->blocks_per_cu = best.bpc;
+	out->target = best.target;
+	out->trials = trials;
+	out->steps_per_s = best_rate;
+	return NBL_OK;
+}
+
+int nbl_driver_set_graph(nbl_driver* d, int enable)
+{
+	clear_error();
+	if (!d)
+		return fail(NBL_ERR_ARG, "null driver");
+	d->use_graph = enable != 0;
+	return NBL_OK;
+}
+
+int nbl_driver_set_launch(nbl_driver* d, uint32_t steps_per_launch, uint32_t blocks_per_cu, uint32_t target)
+{
+	clear_error();
+	if (!d)
+		return fail(NBL_ERR_ARG, "null driver");
+	if (steps_per_launch < 1 || steps_per_launch > 1024)
+		return fail(NBL_ERR_ARG, "steps_per_launch must be in [1, 1024]");
+	if (blocks_per_cu < 1 || blocks_per_cu > 32)
+		return fail(NBL_ERR_ARG, "blocks_per_cu must be in [1, 32]");
+	if (target < 1 || target > d->capacity)
+		return fail(NBL_ERR_ARG, "target must be in [1, capacity]");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	if (d->K != steps_per_launch || d->blocks_per_cu != blocks_per_cu)
+		d->epoch++;
+	d->K = steps_per_launch;
+	d->blocks_per_cu = blocks_per_cu;
+	// the control block is only read by kernels: update it between launches
+	HIP_OK(hipStreamSynchronize(d->stream));
+	HIP_OK(hipMemcpy(&d->ctrl->target, &target, sizeof(uint32_t), hipMemcpyHostToDevice));
+	d->target = target;
 	return NBL_OK;
 }
 
@@ -966,6 +1421,32 @@ int nbl_driver_counts(nbl_driver* d, uint32_t* running, uint32_t* detected, uint
 	return NBL_OK;
 }
 
+namespace {
+// Copy out the detected records (and their scores, if `scores`) and clear
+// the device buffer.
+int drain(nbl_driver* d, nbl_detected* out, double* scores, uint32_t cap, uint32_t* n)
+{
+	int rc = read_ctrl(d);
+	if (rc)
+		return rc;
+	const uint32_t have = std::min(d->h_ctrl->det_ctr, d->det_cap);
+	if (have > cap) {
+		*n = have;
+		return fail(NBL_ERR_CAPACITY, "output array too small for " + std::to_string(have) + " records");
+	}
+	if (have) {
+		HIP_OK(hipMemcpyAsync(out, d->det, sizeof(nbl_detected) * have, hipMemcpyDeviceToHost, d->stream));
+		if (scores && d->map.np)
+			HIP_OK(hipMemcpyAsync(scores, d->det_sc, sizeof(double) * d->map.np * (size_t)have,
+				hipMemcpyDeviceToHost, d->stream));
+	}
+	hipLaunchKernelGGL(k_reset_det, dim3(1), dim3(64), 0, d->stream, d->ctrl);
+	HIP_OK(hipStreamSynchronize(d->stream));
+	*n = have;
+	return NBL_OK;
+}
+} // namespace
+
 int nbl_driver_drain_detected(nbl_driver* d, nbl_detected* out, uint32_t cap, uint32_t* n)
 {
 	clear_error();
@@ -973,19 +1454,20 @@ int nbl_driver_drain_detected(nbl_driver* d, nbl_detected* out, uint32_t cap, ui
 		return fail(NBL_ERR_ARG, "invalid drain arguments");
 	int rc = set_device(d->dev);
 	if (rc) return rc;
-	if ((rc = read_ctrl(d)))
-		return rc;
-	const uint32_t have = std::min(d->h_ctrl->det_ctr, d->det_cap);
-	if (have > cap) {
-		*n = have;
-		return fail(NBL_ERR_CAPACITY, "output array too small for " + std::to_string(have) + " records");
-	}
-	if (have)
-		HIP_OK(hipMemcpyAsync(out, d->det, sizeof(nbl_detected) * have, hipMemcpyDeviceToHost, d->stream));
-	hipLaunchKernelGGL(k_reset_det, dim3(1), dim3(64), 0, d->stream, d->ctrl);
-	HIP_OK(hipStreamSynchronize(d->stream));
-	*n = have;
-	return NBL_OK;
+	return drain(d, out, nullptr, cap, n);
+}
+
+int nbl_driver_drain_detected_scores(nbl_driver* d, nbl_detected* out, double* scores, uint32_t cap,
+	uint32_t* n)
+{
+	clear_error();
+	if (!d || !n || (cap && (!out || !scores)))
+		return fail(NBL_ERR_ARG, "invalid drain arguments");
+	if (!d->map.np)
+		return fail(NBL_ERR_ARG, "scores are not enabled on this driver");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	return drain(d, out, scores, cap, n);
 }
 
 int nbl_driver_steps(nbl_driver* d, uint64_t* steps)
@@ -1001,14 +1483,14 @@ int nbl_driver_steps(nbl_driver* d, uint64_t* steps)
 	return NBL_OK;
 }
 
-int nbl_driver_run(nbl_driver* d, nbl_detected* out, uint64_t cap, uint64_t* n_detected)
+namespace {
+int run(nbl_driver* d, nbl_detected* out, double* scores, uint64_t cap, uint64_t* n_detected)
 {
-	clear_error();
-	if (!d || !n_detected || (cap && !out))
-		return fail(NBL_ERR_ARG, "invalid run arguments");
+	const size_t np = scores ? (size_t)d->map.np : 0;
 	uint64_t total = 0;
 	uint32_t frame = 4;
 	std::vector<nbl_detected> tmp;
+	std::vector<double> tmp_sc;
 	uint64_t last_steps = ~0ull;
 	int stalled = 0;
 	for (;;) {
@@ -1022,14 +1504,18 @@ int nbl_driver_run(nbl_driver* d, nbl_detected* out, uint64_t cap, uint64_t* n_d
 			uint32_t got = 0;
 			const uint64_t room = cap > total ? cap - total : 0;
 			if (room >= det) {
-				if ((rc = nbl_driver_drain_detected(d, out + total, det, &got)))
+				if ((rc = drain(d, out + total, scores ? scores + total * np : nullptr, det, &got)))
 					return rc;
 			} else {
 				tmp.resize(det);
-				if ((rc = nbl_driver_drain_detected(d, tmp.data(), det, &got)))
+				tmp_sc.resize((size_t)det * std::max<size_t>(np, 1));
+				if ((rc = drain(d, tmp.data(), scores ? tmp_sc.data() : nullptr, det, &got)))
 					return rc;
-				if (room)
+				if (room) {
 					std::memcpy(out + total, tmp.data(), sizeof(nbl_detected) * room);
+					if (scores)
+						std::memcpy(scores + total * np, tmp_sc.data(), sizeof(double) * np * room);
+				}
 			}
 			total += got;
 		}
@@ -1050,6 +1536,104 @@ int nbl_driver_run(nbl_driver* d, nbl_detected* out, uint64_t cap, uint64_t* n_d
 	*n_detected = total;
 	if (total > cap)
 		return fail(NBL_ERR_CAPACITY, "more detected electrons than the output array holds");
+	return NBL_OK;
+}
+} // namespace
+
+int nbl_driver_run(nbl_driver* d, nbl_detected* out, uint64_t cap, uint64_t* n_detected)
+{
+	clear_error();
+	if (!d || !n_detected || (cap && !out))
+		return fail(NBL_ERR_ARG, "invalid run arguments");
+	return run(d, out, nullptr, cap, n_detected);
+}
+
+int nbl_driver_run_scores(nbl_driver* d, nbl_detected* out, double* scores, uint64_t cap,
+	uint64_t* n_detected)
+{
+	clear_error();
+	if (!d || !n_detected || (cap && (!out || !scores)))
+		return fail(NBL_ERR_ARG, "invalid run arguments");
+	if (!d->map.np)
+		return fail(NBL_ERR_ARG, "scores are not enabled on this driver");
+	return run(d, out, scores, cap, n_detected);
+}
+
+int nbl_driver_set_material_params(nbl_driver* d, int32_t material, float el_mfp_scale,
+	float in_mfp_scale, float barrier)
+{
+	clear_error();
+	if (!d || material < 0 || material >= (int32_t)d->W.nmats)
+		return fail(NBL_ERR_ARG, "no such material");
+	if (!(el_mfp_scale > 0.0f) || !(in_mfp_scale > 0.0f) || !std::isfinite(el_mfp_scale) ||
+		!std::isfinite(in_mfp_scale) || !std::isfinite(barrier))
+		return fail(NBL_ERR_ARG, "mean-free-path scales must be positive and finite, the barrier finite");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	Material& m = d->h_mats[(size_t)material];
+	m.el_scale = el_mfp_scale;
+	m.in_scale = in_mfp_scale;
+	m.barrier = barrier;
+	HIP_OK(hipMemcpyAsync(d->d_mats + material, &m, sizeof(Material), hipMemcpyHostToDevice, d->stream));
+	HIP_OK(hipStreamSynchronize(d->stream));
+	return NBL_OK;
+}
+
+int nbl_driver_get_material_params(nbl_driver* d, int32_t material, float* el_mfp_scale,
+	float* in_mfp_scale, float* barrier)
+{
+	clear_error();
+	if (!d || material < 0 || material >= (int32_t)d->W.nmats || !el_mfp_scale || !in_mfp_scale || !barrier)
+		return fail(NBL_ERR_ARG, "invalid arguments");
+	const Material& m = d->h_mats[(size_t)material];
+	*el_mfp_scale = m.el_scale;
+	*in_mfp_scale = m.in_scale;
+	*barrier = m.barrier;
+	return NBL_OK;
+}
+
+int nbl_driver_set_score_params(nbl_driver* d, const int32_t* materials, const int32_t* kinds, int32_t n)
+{
+	clear_error();
+	if (!d || n < 0 || n > MAX_PARAMS || (n && (!materials || !kinds)))
+		return fail(NBL_ERR_ARG, "between 0 and 8 parameters");
+	ScoreMap map;
+	map.np = n;
+	std::memset(map.pidx, -1, sizeof(map.pidx));
+	for (int32_t i = 0; i < n; ++i) {
+		if (materials[i] < 0 || materials[i] >= std::min<int32_t>(d->W.nmats, MAX_SCORED_MATERIALS))
+			return fail(NBL_ERR_ARG, "parameter " + std::to_string(i) + ": no such material (at most 16 carry parameters)");
+		if (kinds[i] < 0 || kinds[i] > 2)
+			return fail(NBL_ERR_ARG, "parameter " + std::to_string(i) + ": kind must be 0, 1 or 2");
+		if (map.pidx[materials[i]][kinds[i]] >= 0)
+			return fail(NBL_ERR_ARG, "parameter " + std::to_string(i) + " repeats an earlier one");
+		map.pidx[materials[i]][kinds[i]] = (int8_t)i;
+	}
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	if ((rc = read_ctrl(d)))
+		return rc;
+	if (active_count(*d->h_ctrl, d->spill_cap) != 0 || d->h_ctrl->queue_n != d->h_ctrl->queue_next ||
+		d->h_ctrl->det_ctr != 0)
+		return fail(NBL_ERR_ARG, "scores can only be switched while no particle is in flight or detected");
+	for (double*& p : d->sc_mem) { hipFree(p); p = nullptr; }
+	hipFree(d->det_sc);
+	d->det_sc = nullptr;
+	Bufs* bufs[4] = { &d->buf[0], &d->buf[1], &d->spill, &d->inel };
+	for (Bufs* b : bufs)
+		b->sc = nullptr;
+	d->map.np = 0;
+	d->epoch++;
+	if (n == 0)
+		return NBL_OK;
+	for (int k = 0; k < 4; ++k) {
+		if ((rc = dev_alloc(&d->sc_mem[k], (size_t)n * bufs[k]->sc_stride)))
+			return rc;
+		bufs[k]->sc = d->sc_mem[k];
+	}
+	if ((rc = dev_alloc(&d->det_sc, (size_t)n * d->det_cap)))
+		return rc;
+	d->map = map;
 	return NBL_OK;
 }
 

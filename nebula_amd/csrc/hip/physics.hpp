@@ -33,6 +33,9 @@ enum { ALIVE = 0, TERMINATED = 1, DETECTED = 2 };
 enum { EV_NONE = 0, EV_INELASTIC = 1, EV_ELASTIC = 2, EV_INTERSECT = 3 };
 // Particle flags.
 constexpr uint32_t FL_PEND_INEL = 1u, FL_PEND_DET = 2u;
+// Bits 16..31 of a particle's flags: its world (geometry + material set) in
+// a driver that tracks several scans at once.
+constexpr uint32_t FL_WORLD_SHIFT = 16u, FL_WORLD_MASK = 0xffff0000u;
 
 struct F3 { float x, y, z; };
 
@@ -422,6 +425,8 @@ __device__ __forceinline__ float t3_get_rounddown(const Table3& t, float x, floa
 // -------------------------------------------------------------- material --
 struct Material {
 	float barrier, phonon_loss, recoil_const, fermi, band_gap;
+	// mean-free-path scales of the derivative estimator (1 = the tables)
+	float el_scale, in_scale;
 	int inel_model;
 	Table1 el_imfp, in_imfp;
 	Table2 el_icdf, in_icdf, ki_ion;
@@ -597,13 +602,20 @@ __device__ inline int propagate(const Geometry& g, F3 start, F3 dir, float dista
 // -------------------------------------------------------------- physics --
 struct Particle { F3 pos, dir; float E; };
 
-// physics/full_penn.h:50-57 (same body in elastic.h:54-62)
-__device__ __forceinline__ float mech_sample_path(const Table1& t, float E, Rng& rng)
+// physics/full_penn.h:50-57 (same body in elastic.h:54-62); the mean free
+// path is multiplied by the mechanism's scale (exact when it is 1).
+__device__ __forceinline__ float mech_sample_path(const Table1& t, float scale, float E, Rng& rng,
+	float& tau)
 {
 	const float imfp = m_exp(t1_get(t, m_log(E)));
-	const float tau = 1.0f / imfp;
+	tau = (1.0f / imfp) * scale;
 	return -tau * m_log(rng_unit(rng));
 }
+
+// What the derivative estimator needs from a step's free-path race: the
+// scaled mean free paths, the winning mechanism (0 none, 1 inelastic,
+// 2 elastic) and the distance travelled.
+struct PathInfo { float tau_el, tau_in, dist; int type; };
 
 // physics/kieft/elastic.h:67-115
 __device__ inline void elastic_execute(const Material& m, uint32_t fl, Particle& p, Rng& rng)
@@ -818,9 +830,14 @@ __device__ inline bool kieft_execute(const Material& m, uint32_t fl, Particle& p
 }
 
 // physics/boundary_intersect.h:38-155
+//
+// With `g` given, it receives d log p / d dU of the random choices made
+// (transmission, reflection, absorption; 0 if none), and mi / mo the
+// materials with dU = b(mo) - b(mi).
 __device__ inline int boundary_execute(const Material* mats, uint32_t fl, const Tri& tri,
-	Particle& p, int& mat, Rng& rng)
+	Particle& p, int& mat, Rng& rng, float* g = nullptr, int* g_mi = nullptr, int* g_mo = nullptr)
 {
+	if (g) *g = 0.0f;
 	Particle tp = p;
 	const F3 nd = normalised(tp.dir);
 	const F3 n = normalised(cross(tri.e1, tri.e2));
@@ -849,10 +866,15 @@ __device__ inline int boundary_execute(const Material* mats, uint32_t fl, const 
 	float dU = 0.0f;
 	if (mi >= 0) dU = dU - mats[mi].barrier;
 	if (mo >= 0) dU = dU + mats[mo].barrier;
+	if (g) { *g_mi = mi; *g_mo = mo; }
 	if (tp.E * cos_theta * cos_theta + dU > 0.0f) {
-		const float s = sqrtf(1.0f + dU / (tp.E * cos_theta * cos_theta));
+		const float ec2 = tp.E * cos_theta * cos_theta;
+		const float s = sqrtf(1.0f + dU / ec2);
 		const float T = (fl & PH_BI_QUANTUM) ? 4.0f * s / ((1.0f + s) * (1.0f + s)) : 1.0f;
 		if (rng_unit(rng) < T) {
+			// d log T / d dU
+			if (g && (fl & PH_BI_QUANTUM) && T < 1.0f)
+				*g = (1.0f - s) / (2.0f * s * s * (1.0f + s) * ec2);
 			if (fl & PH_BI_REFRACT)
 				tp.dir = v_add(v_sub(nd, v_mul(n, cos_theta)), v_mul(s_mul(s, n), cos_theta));
 			tp.E = tp.E + dU;
@@ -860,10 +882,19 @@ __device__ inline int boundary_execute(const Material* mats, uint32_t fl, const 
 			p = tp;
 			return ALIVE;
 		}
+		// d log (1 - T) / d dU
+		if (g && (fl & PH_BI_QUANTUM) && T < 1.0f)
+			*g = 2.0f / (s * dU);
 	}
 	if (fl & PH_BI_ABSORB) {
-		if (dU < 0.0f && rng_unit(rng) < m_exp(1.0f + 0.5f * tp.E / dU))
-			return TERMINATED;
+		if (dU < 0.0f) {
+			const float P = m_exp(1.0f + 0.5f * tp.E / dU);
+			const bool absorbed = rng_unit(rng) < P;
+			if (g && P < 1.0f)
+				*g = *g + (absorbed ? -0.5f * tp.E / (dU * dU) : P * 0.5f * tp.E / (dU * dU * (1.0f - P)));
+			if (absorbed)
+				return TERMINATED;
+		}
 	}
 	tp.dir = v_sub(nd, v_mul(s_mul(2.0f, n), cos_theta));
 	p = tp;
@@ -887,18 +918,27 @@ struct World {
 	uint32_t key0, key1;
 };
 
+// Geometry and materials of one world of a multi-scan driver.
+struct WorldDesc {
+	Geometry g;
+	const Material* mats;
+	int nmats;
+};
+
 // init (drivers/gpu/gpu_driver.inl:403-480): domain and energy checks, path
 // sampling, propagate, move; stores the normalised direction.  Returns ALIVE
 // or TERMINATED and the event; `hit` is the intersected triangle.
 __device__ inline int step_init(const World& W, State& s, Rng& rng, int& event, int& hit,
-	int* path, int stride)
+	int* path, int stride, PathInfo& pinfo)
 {
 	event = EV_NONE;
 	hit = -1;
+	pinfo.type = 0;
 	if (!in_domain(W.g, s.p.pos))
 		return TERMINATED;
 	int type = 0;
 	float distance = W.g.max_extent;
+	float tau_el = 0.0f, tau_in = 0.0f;
 	if (s.mat >= 0) {
 		const Material& m = W.mats[s.mat];
 		if (s.p.E < m.barrier + W.threshold)
@@ -906,9 +946,9 @@ __device__ inline int step_init(const World& W, State& s, Rng& rng, int& event, 
 		// core/scatter_list.h:101-107 with the reverse visiting order of
 		// common/tuple.h:114-124: elastic first, strictly shorter wins.
 		float dist = __builtin_inff();
-		const float pe = mech_sample_path(m.el_imfp, s.p.E, rng);
+		const float pe = mech_sample_path(m.el_imfp, m.el_scale, s.p.E, rng, tau_el);
 		if (pe < dist) { type = 2; dist = pe; }
-		const float pi = mech_sample_path(m.in_imfp, s.p.E, rng);
+		const float pi = mech_sample_path(m.in_imfp, m.in_scale, s.p.E, rng, tau_in);
 		if (pi < dist) { type = 1; dist = pi; }
 		distance = dist;
 	}
@@ -924,6 +964,10 @@ __device__ inline int step_init(const World& W, State& s, Rng& rng, int& event, 
 		s.last_tri = hit;
 		event = EV_INTERSECT;
 	}
+	pinfo.tau_el = tau_el;
+	pinfo.tau_in = tau_in;
+	pinfo.dist = (hit < 0) ? distance : hd;
+	pinfo.type = (s.mat >= 0) ? type : 0;
 	return ALIVE;
 }
 
@@ -931,12 +975,12 @@ __device__ inline int step_init(const World& W, State& s, Rng& rng, int& event, 
 // the new parent state and secondary are computed into `s`/`se`; the caller
 // commits them.
 __device__ inline int step_event(const World& W, State& s, Rng& rng, int event, int hit,
-	Particle& se, bool& se_made)
+	Particle& se, bool& se_made, float* g = nullptr, int* g_mi = nullptr, int* g_mo = nullptr)
 {
 	se_made = false;
 	if (event == EV_INTERSECT) {
 		const Tri tri = load_tri(W.g.tris, hit);
-		return boundary_execute(W.mats, W.flags, tri, s.p, s.mat, rng);
+		return boundary_execute(W.mats, W.flags, tri, s.p, s.mat, rng, g, g_mi, g_mo);
 	}
 	if (event == EV_NONE)
 		return ALIVE;

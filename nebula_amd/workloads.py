@@ -232,6 +232,79 @@ def read_tri(path: str) -> Mesh:
                 a[:, 0].astype(np.int32), a[:, 1].astype(np.int32))
 
 
+def contact_hole_geometry(nx: int = 4, ny: int = 4, pitch: float = 80.0, diameter: float = 40.0,
+                          layers=((80.0, 1),), substrate_mat: int = 0, substrate_depth: float = 200.0,
+                          n_seg: int = 32, nz: int = 1, radius_sigma: float = 0.0,
+                          detector_gap: float = 10.0, seed: int = 1) -> Mesh:
+    """An nx x ny array of cylindrical holes (vacuum) through a stack of
+    layers [(thickness, material), ...] from z=0 down, on a substrate.  Each
+    hole is an n_seg-gon (n_seg a multiple of 4) whose vertex radii carry
+    Gaussian jitter `radius_sigma` (seeded), straight through the stack; each
+    layer's sidewall is split into `nz` bands.  Every cell of side `pitch`
+    meshes its faces as a ring between the square cell edge and the hole, so
+    neighbouring cells share edges exactly."""
+    if n_seg % 4:
+        raise ValueError("n_seg must be a multiple of 4")
+    rng = np.random.default_rng(seed)
+    mb = _MeshBuilder()
+    X, Y = nx * pitch / 2, ny * pitch / 2
+    V, S = VACUUM, substrate_mat
+    # layer boundaries: z_top[0] = 0 > ... > z_bottom[-1] = -depth
+    tops, mats_l = [], []
+    z = 0.0
+    for t, m in layers:
+        tops.append((z, z - t))
+        mats_l.append(m)
+        z -= t
+    depth = -z
+    ang = np.pi / 4 + 2 * np.pi * np.arange(n_seg) / n_seg     # includes the cell corners
+    c, sn = np.cos(ang), np.sin(ang)
+    edge = (pitch / 2) / np.maximum(np.abs(c), np.abs(sn))      # ray to the cell edge
+    P = lambda x, y, z: (float(x), float(y), float(z))  # noqa: E731
+    for ix in range(nx):
+        for iy in range(ny):
+            cx, cy = -X + pitch * (ix + 0.5), -Y + pitch * (iy + 0.5)
+            r = diameter / 2 + radius_sigma * rng.standard_normal(n_seg)
+            r = np.clip(r, 0.05 * diameter, 0.45 * pitch)
+            hx, hy = cx + r * c, cy + r * sn
+            bx, by = cx + edge * c, cy + edge * sn
+            k1 = np.roll(np.arange(n_seg), -1)
+            # rings: top face, then each interface below a layer (normal +z,
+            # from the lower material to the upper one)
+            ring_mats = [(mats_l[0], V, 0.0)]
+            for li in range(len(layers)):
+                below = mats_l[li + 1] if li + 1 < len(layers) else S
+                ring_mats.append((below, mats_l[li], tops[li][1]))
+            for m_in, m_out, zz in ring_mats:
+                for k in range(n_seg):
+                    j = k1[k]
+                    mb.quad(P(hx[k], hy[k], zz), P(bx[k], by[k], zz), P(bx[j], by[j], zz),
+                            P(hx[j], hy[j], zz), m_in, m_out)
+            # sidewalls, normal into the hole
+            for li, (z_hi, z_lo) in enumerate(tops):
+                zs = np.linspace(z_lo, z_hi, nz + 1)
+                for b in range(nz):
+                    z0, z1 = zs[b], zs[b + 1]
+                    for k in range(n_seg):
+                        j = k1[k]
+                        mb.quad(P(hx[k], hy[k], z0), P(hx[k], hy[k], z1), P(hx[j], hy[j], z1),
+                                P(hx[j], hy[j], z0), mats_l[li], V)
+            # hole bottom: substrate under vacuum
+            for k in range(n_seg):
+                j = k1[k]
+                mb.tri(P(cx, cy, -depth), P(hx[k], hy[k], -depth), P(hx[j], hy[j], -depth), S, V)
+    # outer faces of every layer and of the substrate, and the substrate bottom
+    D = depth + substrate_depth
+    for (z_hi, z_lo), m in list(zip(tops, mats_l)) + [((-depth, -D), S)]:
+        mb.quad(P(-X, -Y, z_lo), P(-X, -Y, z_hi), P(-X, Y, z_hi), P(-X, Y, z_lo), m, V)
+        mb.quad(P(X, -Y, z_lo), P(X, Y, z_lo), P(X, Y, z_hi), P(X, -Y, z_hi), m, V)
+        mb.quad(P(-X, -Y, z_lo), P(X, -Y, z_lo), P(X, -Y, z_hi), P(-X, -Y, z_hi), m, V)
+        mb.quad(P(-X, Y, z_lo), P(-X, Y, z_hi), P(X, Y, z_hi), P(X, Y, z_lo), m, V)
+    mb.quad(P(-X, -Y, -D), P(-X, Y, -D), P(X, Y, -D), P(X, -Y, -D), S, V)
+    _detector_plane(mb, -X, X, -Y, Y, detector_gap)
+    return mb.mesh()
+
+
 # -------------------------------------------------------------- primaries --
 def pencil_beam(n: int, energy: float = 1000.0, z0: float = 5.0) -> np.ndarray:
     """n primaries at (0, 0, z0) moving along -z, pixel (0, 0)."""
@@ -277,6 +350,42 @@ def read_pri(path: str) -> np.ndarray:
     if raw.size % 36:
         raise ValueError("Unexpected end of primaries file")
     return raw.view(PRI_DTYPE)
+
+
+def batch_scans(n_scans: int = 64, seed: int = 1, per_pixel_scale: float = 1.0):
+    """Independent scans of uneven size: each a seeded random geometry (a line
+    grating or a contact-hole array, 1e3 to 2e5 triangles, log-uniform), a
+    beam energy log-uniform in 200 eV - 20 keV, a raster of 32x32 to 256x256
+    pixels and 10 to 200 primaries per pixel (times `per_pixel_scale`).
+    Materials: 0 = Si substrate, 1 = PMMA (lines or hole layer).  Returns a
+    list of dicts with kind, mesh, primaries, energy, raster side, per_pixel
+    and target triangle count."""
+    rng = np.random.default_rng(seed)
+    scans = []
+    for i in range(n_scans):
+        target = float(np.exp(rng.uniform(np.log(1e3), np.log(2e5))))
+        energy = float(np.exp(rng.uniform(np.log(200.0), np.log(20000.0))))
+        side = int(rng.integers(32, 257))
+        per_px = max(1, int(round(int(rng.integers(10, 201)) * per_pixel_scale)))
+        gseed = int(rng.integers(1 << 30))
+        if rng.random() < 0.5:
+            kind = "grating"
+            n_lines, nz = 10, 4
+            ny = max(2, int(round(target / (n_lines * (4 * nz + 6)))))
+            mesh = grating_geometry(n_lines=n_lines, nz=nz, dy=320.0 / ny, seed=gseed)
+            half, top = 160.0, 32.0
+        else:
+            kind = "holes"
+            nh = int(rng.integers(2, 11))
+            n_seg = max(8, 4 * int(round(target / (9.0 * nh * nh) / 4)))
+            mesh = contact_hole_geometry(nh, nh, pitch=80.0, diameter=40.0, layers=((80.0, 1),),
+                                         n_seg=n_seg, nz=2, radius_sigma=1.0, seed=gseed)
+            half, top = 40.0 * nh, 0.0
+        f = 0.8 * half
+        pri = raster_scan(side, side, per_px, energy, -f, f, -f, f, top + 5.0, spot_sigma=1.0, seed=gseed)
+        scans.append({"kind": kind, "mesh": mesh, "primaries": pri, "energy": energy, "side": side,
+                      "per_pixel": per_px, "target_triangles": int(target)})
+    return scans
 
 
 # -------------------------------------------------------------- materials --

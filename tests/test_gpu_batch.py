@@ -1,0 +1,118 @@
+"""Several scans (worlds: geometry + materials) tracked by one driver.
+
+Every particle keeps its world through its cascade, and its trajectory is
+a function of (seed, stream id) only, so a mixed run must return exactly
+the union of the oracle's per-world runs (records keyed by stream id)."""
+import numpy as np
+import pytest
+
+from nebula_amd import api as A
+from nebula_amd import workloads as W
+from oracle import pyoracle as O
+from tests.helpers import grating_tris, material, slab_tris
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = A.Device(0)
+    yield d
+    d.close()
+
+
+def _keyed(det):
+    return np.sort(det, order="sid")
+
+
+@pytest.mark.parametrize("capacity", [1 << 16, 256])
+def test_mixed_worlds_equal_per_world_oracle(dev, capacity):
+    si, pmma = material("si"), material("pmma")
+    g_slab = A.Geometry(dev, slab_tris())
+    g_grat = A.Geometry(dev, grating_tris())
+    m_si = A.Material(dev, A.HostMaterial(si))
+    m_pm = A.Material(dev, A.HostMaterial(pmma))
+    d = A.Driver(dev, capacity, g_slab, [m_si], seed=13, steps_per_launch=8)
+    w1 = d.add_world(g_grat, [m_si, m_pm])
+    w2 = d.add_world(g_slab, [m_si])
+    assert (w1, w2) == (1, 2)
+    p0 = W.pencil_beam(120, 1000.0)
+    p1 = W.raster_scan(4, 4, 10, 300.0, -32.0, 32.0, -32.0, 32.0, 37.0, seed=2)
+    p2 = W.pencil_beam(80, 2000.0)
+    # interleave pushes; sids are disjoint per scan, tags index the scan's list
+    d.push(p1, np.arange(len(p1), dtype=np.uint32) + 10000, sid_base=1 << 32, world=w1)
+    d.push(p0, np.arange(len(p0), dtype=np.uint32), sid_base=0, world=0)
+    d.push(p2, np.arange(len(p2), dtype=np.uint32) + 20000, sid_base=2 << 32, world=w2)
+    got = d.run(200000)
+    o_slab = O.OracleWorld(slab_tris(), [si])
+    o_grat = O.OracleWorld(grating_tris(), [si, pmma])
+    e0, _ = o_slab.run_philox(p0, 13, 100000, threads=8, sid_base=0, tag_base=0)
+    e1, _ = o_grat.run_philox(p1, 13, 100000, threads=8, sid_base=1 << 32, tag_base=10000)
+    e2, _ = o_slab.run_philox(p2, 13, 100000, threads=8, sid_base=2 << 32, tag_base=20000)
+    exp = np.concatenate([e0, e1, e2])
+    print(f"capacity {capacity}: {len(got)} records (worlds: {len(e0)}, {len(e1)}, {len(e2)})")
+    assert len(e0) > 10 and len(e1) > 10 and len(e2) > 10
+    assert len(got) == len(exp)
+    assert _keyed(got).tobytes() == _keyed(exp).tobytes()
+
+
+def test_bad_world_is_rejected(dev):
+    g = A.Geometry(dev, slab_tris())
+    d = A.Driver(dev, 1024, g, [A.Material(dev, A.HostMaterial(material("si")))], seed=1)
+    with pytest.raises(A.NebulaError):
+        d.push(W.pencil_beam(4, 1000.0), world=1)
+
+
+def test_graph_replay_equals_eager(dev):
+    """Iterations replayed from captured graphs (several frame lengths, each
+    replayed many times with drains in between) give the eager run's
+    records."""
+    g = A.Geometry(dev, grating_tris())
+    mats = [A.Material(dev, A.HostMaterial(material(m))) for m in ("si", "pmma")]
+    pri = W.raster_scan(8, 8, 20, 300.0, -64.0, 64.0, -64.0, 64.0, 37.0, seed=6)
+    tags = np.arange(len(pri), dtype=np.uint32)
+    out = []
+    for graph in (False, True, True):
+        d = A.Driver(dev, 2048, g, mats, seed=3, steps_per_launch=8)
+        d.set_graph(graph)
+        d.push(pri, tags)
+        out.append(_keyed(d.run(100000)))
+    print(f"graph replay: {len(out[0])} records")
+    assert len(out[0]) > 100
+    assert out[0].tobytes() == out[1].tobytes() == out[2].tobytes()
+
+
+def test_autotune_keeps_results(dev):
+    """The autotuner's trials are real work: after tuning and the rest of
+    the run, the records are the oracle's."""
+    g = A.Geometry(dev, grating_tris())
+    si, pmma = material("si"), material("pmma")
+    mats = [A.Material(dev, A.HostMaterial(m)) for m in (si, pmma)]
+    pri = W.raster_scan(16, 16, 16, 300.0, -64.0, 64.0, -64.0, 64.0, 37.0, seed=7)
+    tags = np.arange(len(pri), dtype=np.uint32)
+    d = A.Driver(dev, 1024, g, mats, seed=5, steps_per_launch=8)
+    d.push(pri, tags)
+    r = d.autotune(iters_per_trial=2)
+    print(f"autotune: {r}")
+    assert r["steps_per_launch"] in (4, 8, 16, 32) and r["blocks_per_cu"] in (4, 5, 6, 8)
+    assert 64 <= r["target"] <= 1024 and r["trials"] >= 2 and r["steps_per_s"] > 0
+    got = _keyed(d.run(200000))
+    ow = O.OracleWorld(grating_tris(), [si, pmma])
+    exp, _ = ow.run_philox(pri, 5, 200000, threads=8)
+    assert got.tobytes() == _keyed(exp).tobytes()
+
+
+def test_batched_scans_equal_plain_scans(dev):
+    """A reduced version of the 64-scan batch (6 scans, uneven sizes and
+    energies, gratings and contact holes): the batched run (worlds, graphs,
+    autotuned launch) and the scans run one by one give identical per-scan
+    SE and BSE counts."""
+    from nebula_amd import batch as B
+    scans = W.batch_scans(6, seed=3, per_pixel_scale=0.02)
+    mats = [A.Material(dev, A.HostMaterial(material(m))) for m in ("si", "pmma")]
+    cb, tb, tune = B.run_batched(dev, scans, mats, capacity=1 << 18, seed=2, iters_per_trial=2)
+    cp, tp = B.run_plain(dev, scans, mats, capacity=1 << 18, seed=2)
+    n = sum(len(s["primaries"]) for s in scans)
+    print(f"batch of 6: {n} primaries, batched {tb:.3f} s, plain {tp:.3f} s, tune {tune}")
+    assert np.all(cb.sum(axis=1) > 0)
+    assert np.array_equal(cb, cp)
