@@ -1586,6 +1586,7 @@ static int boundary_execute_scored(const orc_world* w, const orc_triangle* tri,
 	if (mi >= 0) dU -= w->materials[mi].barrier;
 	if (mo >= 0) dU += w->materials[mo].barrier;
 
+	float g = 0.0f;   /* d log p / d dU of the random choices below */
 	if (tp.E * cos_theta * cos_theta + dU > 0.0f) {
 		const float ec2 = tp.E * cos_theta * cos_theta;
 		const float s = sqrtf(1.0f + dU / ec2);
@@ -1601,19 +1602,21 @@ static int boundary_execute_scored(const orc_world* w, const orc_triangle* tri,
 			return ORC_ALIVE;
 		}
 		if (w->opts.bi_quantum_transmission && T < 1.0f)
-			add_barrier_score(map, score, mi, mo, 2.0f / (s * dU));
+			g = 2.0f / (s * dU);
 	}
 	if (w->opts.bi_interface_absorption) {
 		if (dU < 0.0f) {
 			const float P = m_exp(1.0f + 0.5f * tp.E / dU);
 			const int absorbed = orc_rng_unit(rng) < P;
 			if (P < 1.0f)
-				add_barrier_score(map, score, mi, mo, absorbed ? -0.5f * tp.E / (dU * dU)
-					: P * 0.5f * tp.E / (dU * dU * (1.0f - P)));
-			if (absorbed)
+				g = g + (absorbed ? -0.5f * tp.E / (dU * dU) : P * 0.5f * tp.E / (dU * dU * (1.0f - P)));
+			if (absorbed) {
+				add_barrier_score(map, score, mi, mo, g);
 				return ORC_TERMINATED;
+			}
 		}
 	}
+	add_barrier_score(map, score, mi, mo, g);
 	tp.dir = v_sub(nd, v_mul(s_mul(2.0f, n), cos_theta));
 	*p = tp;
 	return ORC_ALIVE;
