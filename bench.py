@@ -19,6 +19,18 @@ oracle/_ref/) on a bounded crop of the same scan, timed on this host's cores
 (falls back to the oracle's threaded CPU restatement if that binary cannot
 run here).  It runs before the GPU is touched.
 
+The two other parts of the metric are reported beside it:
+  batched - 64 independent scans of uneven size (seeded random gratings and
+            contact-hole arrays, 1e3-2e5 triangles, 200 eV-20 keV, 32^2-256^2
+            pixels at 10-200 primaries per pixel) tracked together by one
+            driver (one world per scan, graph-replayed iterations, autotuned
+            launch) against the same scans run one after another on the
+            plain single-scan path; per-scan yields must be identical.
+  grad    - the grating at 64x64 pixels with ~1e6 primaries: forward run vs
+            the likelihood-ratio replay that yields d(per-pixel SE
+            yield)/d(theta) for 5 parameters (Si and PMMA elastic/inelastic
+            mean-free-path scales, PMMA barrier).
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -124,6 +136,61 @@ def cpu_baseline(mesh, pri_full, mats, target_s=20.0, threads=None):
             "sample": f"{side}x{side}-pixel crop ({len(sample)} primaries), oracle Philox driver"}
 
 
+def bench_batched(dev, scale=1.0):
+    """64 uneven scans: batched (one driver, worlds, graphs, autotune) vs the
+    plain single-scan path run scan by scan."""
+    from nebula_amd import batch as B
+    t0 = time.perf_counter()
+    scans = W.batch_scans(64, seed=1, per_pixel_scale=scale)
+    print(f"batched workload generated", file=sys.stderr, flush=True)
+    gen_s = time.perf_counter() - t0
+    mats = [A.Material(dev, A.HostMaterial(m)) for m in (W.make_material(W.SI, "penn"),
+                                                          W.make_material(W.PMMA, "penn"))]
+    n = int(sum(len(s["primaries"]) for s in scans))
+    print(f"batched workload: {n} primaries, {sum(s['mesh'].n for s in scans)} triangles "
+          f"(generated in {gen_s:.1f} s)", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    geoms = B.geometries(dev, scans)
+    print(f"octrees built and uploaded in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    cb, tb, tune = B.run_batched(dev, scans, mats, capacity=1 << 21, seed=1, geoms=geoms, progress_s=20.0)
+    print(f"batched: {tb:.2f} s, tune {tune}", file=sys.stderr, flush=True)
+
+    def progress(i, t):
+        if i % 8 == 7:
+            print(f"plain: {i + 1} scans, {t:.1f} s", file=sys.stderr, flush=True)
+    cp, tp = B.run_plain(dev, scans, mats, capacity=1 << 21, seed=1, geoms=geoms, progress=progress)
+    print(f"plain: {tp:.2f} s", file=sys.stderr, flush=True)
+    return {"workload": "64 scans: seeded gratings / contact holes, 1e3-2e5 triangles, 200 eV-20 keV, "
+                        "32^2-256^2 px, 10-200 primaries/px" + (f" x {scale}" if scale != 1.0 else ""),
+            "primaries": n, "batched_s": tb, "plain_s": tp,
+            "batched_primaries_per_s": n / tb, "plain_primaries_per_s": n / tp,
+            "speedup": tp / tb, "identical_yields": bool(np.array_equal(cb, cp)), "tune": tune}
+
+
+def bench_grad(dev):
+    """Forward pass vs likelihood-ratio replay on the 64x64-pixel grating
+    with ~1e6 primaries and 5 parameters."""
+    from nebula_amd import grad as G
+    mesh, pri, mats = workload(64, 64, 244)
+    g = A.Geometry(dev, mesh)
+    gm = [A.Material(dev, A.HostMaterial(m)) for m in mats]
+    d = A.Driver(dev, 1 << 21, g, gm, seed=1, steps_per_launch=8)
+    params = [(0, 0), (0, 1), (1, 0), (1, 1), (1, 2)]
+    scan = G.DifferentiableScan(d, pri, params)
+    theta = scan.theta0()
+    scan.forward(theta)            # warm-up
+    y = scan.forward(theta)
+    J = scan.jacobian(theta)
+    t = scan.last_times
+    total = (J.numpy() * scan.per_pixel[:, None]).sum(axis=0) / len(pri)
+    return {"workload": "grating 64x64 px x 244 primaries at 300 eV; d(SE yield)/d(Si el/in MFP scale, "
+                        "PMMA el/in MFP scale, PMMA barrier)",
+            "primaries": len(pri), "forward_s": t["forward_s"], "backward_s": t["backward_s"],
+            "grad_over_fwd": t["backward_s"] / t["forward_s"],
+            "mean_se_yield": float((y.numpy() * scan.per_pixel).sum() / len(pri)),
+            "d_mean_se_yield": [float(v) for v in total]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +204,11 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the run, copy 1 GiB device-to-device with a plain copy kernel "
                          "(reference point for the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-scan batched comparison")
+    ap.add_argument("--batch-scale", type=float, default=0.1,
+                    help="primaries-per-pixel multiplier of the batched workload (1 = the full 64-scan "
+                         "batch, 1.4e8 primaries; the default keeps the bench within minutes)")
+    ap.add_argument("--no-grad", action="store_true", help="skip the gradient cost measurement")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "step_pair_traffic.json"),
                     help="measured HBM bytes per k_step + k_inel launch pair (from rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -190,6 +262,8 @@ def main():
                     ("steps_per_launch", args.steps_per_launch)))
         if same:
             traffic = tr["bytes_per_launch"]
+    batched = None if args.no_batched else bench_batched(dev, args.batch_scale)
+    grad = None if args.no_grad else bench_grad(dev)
     value = n_prim / t
     e = det["E"]
     bse = float(np.sum(e >= 50) / n_prim)
@@ -220,6 +294,8 @@ def main():
                      "alg_bytes_per_launch": b_alg / max(launches, 1)},
         "cpu_baseline": cpu,
         "particle_steps_per_primary": n_steps / n_prim,
+        "batched": batched,
+        "grad": grad,
         "yields": {"bse": bse, "se": se, "detected": int(len(det))},
     }
     print(json.dumps(out))

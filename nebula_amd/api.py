@@ -486,11 +486,16 @@ class Driver:
         check(lib().nbl_driver_run_scores(self._h, ptr(out), ptr(sc), cap, C.byref(n)))
         return out[: n.value], sc[: n.value]
 
-    def run_tally(self, tag_bin: np.ndarray, n_bins: int, frame: int = 32, se_cutoff: float = 50.0):
+    def run_tally(self, tag_bin: np.ndarray, n_bins: int, frame: int = 32, se_cutoff: float = 50.0,
+                  progress_s: float = 0.0):
         """Track everything pushed to the end without keeping the records:
         counts[bin, 0] = detected electrons below `se_cutoff` eV (SE) and
         counts[bin, 1] = the others (BSE), binned by tag_bin[tag].  The
-        detected buffer is drained after every `frame` iterations."""
+        detected buffer is drained after every `frame` iterations; with
+        progress_s > 0 a progress line goes to stderr that often."""
+        import sys
+        import time
+        t_last = time.perf_counter()
         counts = np.zeros((n_bins, 2), np.int64)
         buf = np.zeros(max(1, self.capacity), dtype=DET_DTYPE)
         n = C.c_uint32()
@@ -507,6 +512,9 @@ class Driver:
                 counts[:, 1] += np.bincount(b[~se], minlength=n_bins)
             if running == 0 and queued == 0 and det == 0:
                 return counts
+            if progress_s > 0 and time.perf_counter() - t_last > progress_s:
+                t_last = time.perf_counter()
+                print(f"  tracking: {running} in flight, {queued} queued", file=sys.stderr, flush=True)
             st = self.steps()
             stalled = stalled + 1 if (st == last_steps and det == 0) else 0
             last_steps = st

@@ -26,12 +26,18 @@ def _tags(scans):
     return offs, scan_of_tag
 
 
+def geometries(dev: A.Device, scans):
+    """Octree build and upload of every scan's mesh (host work, untimed)."""
+    return [A.Geometry(dev, s["mesh"]) for s in scans]
+
+
 def run_batched(dev: A.Device, scans, materials, capacity: int = 1 << 21, seed: int = 1,
-                autotune: bool = True, graph: bool = True, iters_per_trial: int = 4):
+                autotune: bool = True, graph: bool = True, iters_per_trial: int = 4, geoms=None,
+                progress_s: float = 0.0):
     """All scans in one driver.  Returns (counts [n_scans, 2] of SE / BSE
     detections, seconds of tracking, tuning result or None).  Geometry and
     material upload and the primaries' transfer to HBM precede the clock."""
-    geoms = [A.Geometry(dev, s["mesh"]) for s in scans]
+    geoms = geoms or geometries(dev, scans)
     d = A.Driver(dev, capacity, geoms[0], materials, seed=seed, steps_per_launch=8)
     worlds = [0] + [d.add_world(g, materials) for g in geoms[1:]]
     offs, scan_of_tag = _tags(scans)
@@ -43,7 +49,7 @@ def run_batched(dev: A.Device, scans, materials, capacity: int = 1 << 21, seed: 
     dev.synchronize()
     t0 = time.perf_counter()
     tune = d.autotune(iters_per_trial) if autotune else None
-    counts = d.run_tally(scan_of_tag, len(scans))
+    counts = d.run_tally(scan_of_tag, len(scans), progress_s=progress_s)
     dev.synchronize()
     t = time.perf_counter() - t0
     d.close()
@@ -51,15 +57,16 @@ def run_batched(dev: A.Device, scans, materials, capacity: int = 1 << 21, seed: 
 
 
 def run_plain(dev: A.Device, scans, materials, capacity: int = 1 << 21, seed: int = 1,
-              steps_per_launch: int = 8):
+              steps_per_launch: int = 8, geoms=None, progress=None):
     """The scans one at a time, each on its own single-world driver with the
     default launch configuration.  Returns (counts, seconds of tracking summed
     over the scans)."""
+    geoms = geoms or geometries(dev, scans)
     offs, scan_of_tag = _tags(scans)
     counts = np.zeros((len(scans), 2), np.int64)
     total = 0.0
     for i, s in enumerate(scans):
-        g = A.Geometry(dev, s["mesh"])
+        g = geoms[i]
         d = A.Driver(dev, capacity, g, materials, seed=seed, steps_per_launch=steps_per_launch)
         n = len(s["primaries"])
         d.push(s["primaries"], np.arange(offs[i], offs[i] + n, dtype=np.uint32), sid_base=i << 40)
@@ -69,4 +76,6 @@ def run_plain(dev: A.Device, scans, materials, capacity: int = 1 << 21, seed: in
         dev.synchronize()
         total += time.perf_counter() - t0
         d.close()
+        if progress:
+            progress(i, total)
     return counts, total

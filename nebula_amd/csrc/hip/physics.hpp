@@ -241,9 +241,14 @@ __device__ __forceinline__ F3 normalised(F3 a)
 // vector_math.h:171-186
 __device__ inline F3 make_normal_vec(F3 dir, float phi)
 {
-	const float az = m_atan2(dir.y, dir.x);
-	float sin_az, cos_az;
-	pf_sincos(az, sin_az, cos_az);
+	// cos and sin of the azimuth atan2(y, x) as x/r and y/r (equal up to
+	// rounding; r = 0 takes the atan2 limits: cosine +-1 by the sign of x)
+	const float r = sqrtf(dir.x * dir.x + dir.y * dir.y);
+	float sin_az = 0.0f, cos_az = signbit(dir.x) ? -1.0f : 1.0f;
+	if (r > 0.0f) {
+		cos_az = dir.x / r;
+		sin_az = dir.y / r;
+	}
 	const F3 unit_v = { dir.z * cos_az, dir.z * sin_az, -sqrtf(dir.x * dir.x + dir.y * dir.y) };
 	const F3 unit_u = cross(unit_v, dir);
 	float sin_phi, cos_phi;
@@ -453,6 +458,9 @@ __device__ __forceinline__ Tri load_tri(const float4* t, int i)
 struct Geometry {
 	const int* od;
 	const float4* tris;
+	// per triangle: e1 x e2 (unnormalised, as the walk computes it) and the
+	// two materials as int16 halves (in: low, out: high)
+	const float4* tside;
 	F3 center, halfsize;
 	float max_extent;
 	int ntri;
@@ -561,10 +569,15 @@ __device__ inline int propagate(const Geometry& g, F3 start, F3 dir, float dista
 				break;
 			if (ti == ignore_tri)
 				continue;
-			const Tri tri = load_tri(g.tris, ti);
-			const int mat_out = (dot(cross(tri.e1, tri.e2), dir) < 0.0f) ? tri.mat_in : tri.mat_out;
+			// octree.inl:128-138: the side the ray leaves into; skip
+			// triangles whose outgoing material is the current one
+			const float4 sd = g.tside[ti];
+			const int mats = __float_as_int(sd.w);
+			const int mat_out = (sd.x * dir.x + sd.y * dir.y + sd.z * dir.z < 0.0f)
+				? (int)(short)(mats & 0xffff) : (mats >> 16);
 			if (mat_out == ignore_mat)
 				continue;
+			const Tri tri = load_tri(g.tris, ti);
 			const float t = intersect_ray(tri, cur, dir);
 			if ((t > 0.0f) && (t <= isect + EPS)) {
 				isect = t;

@@ -293,11 +293,24 @@ static inline orc_vec3 normalised(orc_vec3 a)
 	const float r = 1.0f / sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
 	return v_mul(a, r);
 }
-/* vector_math.h:171-186 */
+/* vector_math.h:171-186.  Under the portable-float policy (the HIP
+ * kernels' arithmetic) the cosine and sine of the azimuth atan2(y, x) are
+ * taken as x/r and y/r with r = sqrt(x^2 + y^2), equal to the
+ * transcendental form up to rounding; r = 0 takes the atan2 limits (cosine
+ * +-1 by the sign of x, sine 0). */
 static orc_vec3 make_normal_vec(orc_vec3 dir, float phi)
 {
+#ifdef ORC_MATH_PF
+	const float r = sqrtf(dir.x * dir.x + dir.y * dir.y);
+	float sin_az = 0.0f, cos_az = signbit(dir.x) ? -1.0f : 1.0f;
+	if (r > 0.0f) {
+		cos_az = dir.x / r;
+		sin_az = dir.y / r;
+	}
+#else
 	const float az = m_atan2(dir.y, dir.x);
 	const float sin_az = m_sin(az), cos_az = m_cos(az);
+#endif
 	const orc_vec3 unit_v = { dir.z * cos_az, dir.z * sin_az, -sqrtf(dir.x * dir.x + dir.y * dir.y) };
 	const orc_vec3 unit_u = cross(unit_v, dir);
 	const float sin_phi = m_sin(phi), cos_phi = m_cos(phi);
