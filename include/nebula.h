@@ -237,6 +237,23 @@ int nbl_driver_push_world(nbl_driver* d, const nbl_particle28* particles, const 
  * driver's stream without waiting. */
 int nbl_driver_iterate(nbl_driver* d, uint32_t n_iter);
 
+/* Energy deposits (drivers/cpu/energydeposit_cpu_driver.h:44-93): one
+ * record per scattering event (elastic or inelastic, not a boundary) that
+ * changes the electron's energy: position after the event, energy before
+ * it, energy lost, tag of the primary, stream id of the electron.  Enable
+ * before tracking; every iteration can write up to (K + 1) records per
+ * particle in flight, so drain after every nbl_driver_iterate call (a drain
+ * that finds records were dropped returns NBL_ERR_CAPACITY). */
+typedef struct {
+	float pos[3];
+	float energy;
+	float deposit;
+	uint32_t tag;
+	uint64_t sid;
+} nbl_deposit;
+int nbl_driver_set_deposits(nbl_driver* d, int enable);
+int nbl_driver_drain_deposits(nbl_driver* d, nbl_deposit* out, uint64_t cap, uint64_t* n);
+
 /* Launch configuration: steps each particle takes per k_step launch,
  * k_step blocks (256 threads) per compute unit, and particles in flight to
  * aim for (<= capacity; the reference's batch sizing, gpu_main.cu:167-175). */
@@ -249,7 +266,7 @@ int nbl_driver_set_graph(nbl_driver* d, int enable);
 
 /* The prescan autotune of gpu_main.cu:136-175 made a measured search:
  * coordinate search over steps per launch {4,8,16,32}, blocks per compute
- * unit {4,5,6,8} and particles in flight {capacity/4, /2, 1}, each trial
+ * unit {4,8,16,32} and particles in flight {capacity/4, /2, 1}, each trial
  * `iters_per_trial` iterations of the queued primaries (real work: their
  * detected electrons stay in the buffer for the next drain).  Leaves the
  * fastest configuration (particle-steps/s) set on the driver. */

@@ -97,6 +97,9 @@ TRI_DTYPE = np.dtype([("mat_in", "<i4"), ("mat_out", "<i4"), ("r0", "<f4", 3),
 PART_DTYPE = np.dtype([("pos", "<f4", 3), ("dir", "<f4", 3), ("E", "<f4")])
 DET_DTYPE = np.dtype([("pos", "<f4", 3), ("dir", "<f4", 3), ("E", "<f4"), ("tag", "<u4"),
                       ("sid", "<u8")])
+DEP_DTYPE = np.dtype([("pos", "<f4", 3), ("energy", "<f4"), ("deposit", "<f4"), ("tag", "<u4"),
+                      ("sid", "<u8")])
+assert DEP_DTYPE.itemsize == 32
 STATE_DTYPE = np.dtype([("pos", "<f4", 3), ("dir", "<f4", 3), ("E", "<f4"), ("mat", "<i4"),
                         ("last_tri", "<i4"), ("tag", "<u4"), ("step", "<u4"), ("flags", "<u4"),
                         ("sid", "<u8")])
@@ -138,6 +141,8 @@ _SIGS = {
     "nbl_driver_push": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint64,
                                   _p(C.c_uint32)]),
     "nbl_driver_iterate": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "nbl_driver_set_deposits": (C.c_int, [C.c_void_p, C.c_int]),
+    "nbl_driver_drain_deposits": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, _p(C.c_uint64)]),
     "nbl_driver_set_launch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
     "nbl_driver_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "nbl_driver_autotune": (C.c_int, [C.c_void_p, C.c_uint32, _p(TuneResult)]),

@@ -400,6 +400,38 @@ class Driver:
                                           world, C.byref(pushed)))
         return pushed.value
 
+    def set_deposits(self, enable: bool) -> None:
+        """Energy-deposit records of every scattering event that changes an
+        electron's energy (drivers/cpu/energydeposit_cpu_driver.h)."""
+        check(lib().nbl_driver_set_deposits(self._h, int(bool(enable))))
+        self.deposits = bool(enable)
+
+    def drain_deposits(self) -> np.ndarray:
+        n = C.c_uint64()
+        rc = lib().nbl_driver_drain_deposits(self._h, None, 0, C.byref(n))
+        if rc == _lib.NBL_OK:
+            return np.zeros(0, dtype=_lib.DEP_DTYPE)
+        if rc != _lib.NBL_ERR_CAPACITY or n.value == 0:
+            check(rc)
+        out = np.zeros(n.value, dtype=_lib.DEP_DTYPE)
+        check(lib().nbl_driver_drain_deposits(self._h, ptr(out), len(out), C.byref(n)))
+        return out[: n.value]
+
+    def run_with_deposits(self, cap: int):
+        """Track everything pushed to the end, one iteration at a time,
+        draining deposits after each: (detected, deposits)."""
+        dets, deps = [], []
+        while True:
+            self.do_iteration(1)
+            deps.append(self.drain_deposits())
+            running, det, queued = self.counts()
+            if det:
+                dets.append(self.flush_detected())
+            if running == 0 and queued == 0:
+                break
+        return (np.concatenate(dets) if dets else np.zeros(0, DET_DTYPE),
+                np.concatenate(deps) if deps else np.zeros(0, _lib.DEP_DTYPE))
+
     def set_launch(self, steps_per_launch: int, blocks_per_cu: int, target: int) -> None:
         check(lib().nbl_driver_set_launch(self._h, steps_per_launch, blocks_per_cu, target))
 

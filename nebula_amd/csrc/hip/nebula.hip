@@ -64,8 +64,8 @@ struct Ctrl {
 	uint32_t spill_ctr;     // spill stack top during a k_step (may overshoot)
 	uint32_t spill_take, spill_src;  // refill from the spill stack
 	uint32_t inel_ctr;      // particles queued for their inelastic event
-	uint32_t work_ctr;      // next input particle for a k_step lane
-	uint32_t pad1;
+	uint32_t work_ctr;      // unused
+	uint32_t dep_ctr;       // energy-deposit records written (may overshoot dep_cap)
 	uint64_t refill_src;
 	uint64_t queue_n, queue_next;
 	uint64_t steps;         // particle-steps (init calls on active particles)
@@ -80,6 +80,8 @@ struct StepArgs {
 	Ctrl* ctrl;
 	nbl_detected* det;
 	double* det_sc;         // scores of detected records, [record][parameter]
+	nbl_deposit* dep;       // energy-deposit records, or null
+	uint32_t dep_cap;
 	ScoreMap map;
 	uint32_t det_cap;
 	uint32_t slots;
@@ -314,19 +316,39 @@ __device__ __forceinline__ bool place_secondary(const StepArgs& a, Ctrl* C, cons
 // code runs in k_inel on waves where every lane needs it (about 7% of the
 // steps of a 300 eV scan would otherwise put it into nearly every wave).
 //
-// Lanes are persistent: a lane takes an input particle (wave-wide ballot,
-// one atomic per wave), steps it until it leaves (detected, terminated,
-// inelastic event) or has taken K steps, writes it out and takes the next
-// one.  A particle that leaves early therefore frees its lane at once
-// instead of idling it for the rest of the K steps.
+// Grid-stride over the input particles; each lane steps its particle up to
+// K times, stopping early when it leaves (detected, terminated, inelastic
+// event), and writes it out once: detected record, inelastic queue or
+// survivor buffer, each kind with one ballot and one atomic per wave.
+// (Lanes that took new work after every step or every few steps were
+// measured slower: the extra ballots and atomics cost more than the idle
+// lanes they saved; the limit is divergence within a step.)
 //
 // SC: also accumulate each particle's likelihood-ratio score (free-path
 // race and boundary choices) and write it with the particle.
 // MULTI: particles belong to different worlds (flags bits 16..31); each lane
 // takes its particle's geometry and materials from the world table.
-template <bool SC, bool MULTI>
-__global__ __launch_bounds__(256) void k_step(StepArgs a)
+// An energy-deposit record (energydeposit_cpu_driver.h:55-66): position
+// after the event, energy before, energy lost; slots by ballot + atomic.
+__device__ __forceinline__ void deposit(const StepArgs& a, Ctrl* C, bool rec, const State& s, float e0)
 {
+	const uint32_t slot = wave_alloc(&C->dep_ctr, rec);
+	if (rec && slot < a.dep_cap) {
+		nbl_deposit r;
+		r.pos[0] = s.p.pos.x; r.pos[1] = s.p.pos.y; r.pos[2] = s.p.pos.z;
+		r.energy = e0;
+		r.deposit = e0 - s.p.E;
+		r.tag = s.tag;
+		r.sid = s.sid;
+		a.dep[slot] = r;
+	}
+}
+
+template <bool SC, bool MULTI>
+__device__ __forceinline__ void step_body(const StepArgs& a)
+{
+	// energy deposits are recorded by the general variants only
+	constexpr bool DEP_OK = SC || MULTI;
 	Ctrl* C = a.ctrl;
 	const uint32_t bi = C->buf_in;
 	const Bufs in = bi ? a.buf[1] : a.buf[0];
@@ -339,72 +361,65 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
 	__shared__ int s_path[PATH_LEVELS * 256];
 	int* path = s_path + threadIdx.x;
 	uint32_t my_steps = 0;
-	State s;
-	Score sc;
-	bool have = false, drained = false;
-	uint32_t k = 0;
-	for (;;) {
-		const bool want = !have && !drained;
-		const uint32_t i = wave_alloc(&C->work_ctr, want);
-		if (want) {
-			if (i < n_in) {
-				const uint32_t slot = (i < n_surv) ? i : (B - 1u - (i - n_surv));
-				s = load_state(in, slot);
-				if (SC) load_score(in, slot, np, sc);
-				if (MULTI) {
-					const WorldDesc& wd = a.worlds[s.flags >> FL_WORLD_SHIFT];
-					Wl.g = wd.g;
-					Wl.mats = wd.mats;
-					Wl.nmats = wd.nmats;
-				}
-				have = true;
-				k = 0;
-			} else {
-				drained = true;
-			}
+	const uint32_t stride = gridDim.x * blockDim.x;
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_in; i += stride) {
+		const uint32_t slot = (i < n_surv) ? i : (B - 1u - (i - n_surv));
+		State s = load_state(in, slot);
+		Score sc;
+		if (SC) load_score(in, slot, np, sc);
+		if (MULTI) {
+			const WorldDesc& wd = a.worlds[s.flags >> FL_WORLD_SHIFT];
+			Wl.g = wd.g;
+			Wl.mats = wd.mats;
+			Wl.nmats = wd.nmats;
 		}
-		if (!__any(have))
-			break;
-		if (!have)
-			continue;
-		int oc = ALIVE;
-		bool to_inel = false;
-		if (s.flags & FL_PEND_DET) {
-			oc = DETECTED;
-		} else if (s.flags & FL_PEND_INEL) {
-			// an inelastic event that found no secondary slot earlier
-			s.flags &= ~FL_PEND_INEL;
-			to_inel = true;
-		} else {
-			int ev = EV_NONE, hit = -1;
-			Rng rng;
-			rng_start(rng, W.key0, W.key1, s.sid, s.step);
-			const int mat0 = s.mat;
-			PathInfo pinfo;
-			oc = step_init(W, s, rng, ev, hit, path, 256, pinfo);
-			++my_steps;
-			if (SC && oc == ALIVE)
-				score_free_path(sc, a.map, W, mat0, pinfo, ev);
-			if (oc == ALIVE) {
-				if (ev == EV_INELASTIC) {
-					to_inel = true;
-				} else {
-					Particle unused;
-					bool made = false;
-					if (SC) {
-						float g = 0.0f;
-						int gmi = -1, gmo = -1;
-						oc = step_event(W, s, rng, ev, hit, unused, made, &g, &gmi, &gmo);
-						if (ev == EV_INTERSECT)
-							score_barrier(sc, a.map, gmi, gmo, g);
+		bool det = false, to_inel = false, dead = false;
+		for (uint32_t k = 0; k < K; ++k) {
+			int oc = ALIVE;
+			if (s.flags & FL_PEND_DET) {
+				oc = DETECTED;
+			} else if (s.flags & FL_PEND_INEL) {
+				// an inelastic event that found no secondary slot earlier
+				s.flags &= ~FL_PEND_INEL;
+				to_inel = true;
+			} else {
+				int ev = EV_NONE, hit = -1;
+				Rng rng;
+				rng_start(rng, W.key0, W.key1, s.sid, s.step);
+				const int mat0 = s.mat;
+				PathInfo pinfo;
+				oc = step_init(W, s, rng, ev, hit, path, 256, pinfo);
+				++my_steps;
+				if (SC && oc == ALIVE)
+					score_free_path(sc, a.map, W, mat0, pinfo, ev);
+				if (oc == ALIVE) {
+					if (ev == EV_INELASTIC) {
+						to_inel = true;
 					} else {
-						oc = step_event(W, s, rng, ev, hit, unused, made);
+						Particle unused;
+						bool made = false;
+						if (SC) {
+							float g = 0.0f;
+							int gmi = -1, gmo = -1;
+							oc = step_event(W, s, rng, ev, hit, unused, made, &g, &gmi, &gmo);
+							if (ev == EV_INTERSECT)
+								score_barrier(sc, a.map, gmi, gmo, g);
+						} else {
+							const float e0 = s.p.E;
+							oc = step_event(W, s, rng, ev, hit, unused, made);
+							if (DEP_OK && a.dep)
+								deposit(a, C, ev == EV_ELASTIC && s.p.E != e0, s, e0);
+						}
 					}
 				}
 			}
+			if (oc == DETECTED) { det = true; break; }
+			if (oc == TERMINATED) { dead = true; break; }
+			if (to_inel)
+				break;
+			s.step++;
 		}
-		bool keep = true, stop = false;
-		const bool det = (oc == DETECTED);
+		bool keep = !dead;
 		const uint32_t dslot = wave_alloc(&C->det_ctr, det);
 		if (det) {
 			if (dslot < det_cap) {
@@ -423,33 +438,41 @@ __global__ __launch_bounds__(256) void k_step(StepArgs a)
 				}
 				keep = false;
 			} else {
+				// no record slot: wait as a survivor and retry next launch
 				s.flags |= FL_PEND_DET;
-				stop = true;
 			}
 		}
-		if (oc == TERMINATED)
-			keep = false;
-		if (keep && !stop && !to_inel) {
-			s.step++;
-			++k;
-		}
-		const bool fin = !keep || stop || to_inel || k == K;
-		const bool queue = fin && keep && to_inel;
+		const bool queue = keep && to_inel;
 		const uint32_t qslot = wave_alloc(&C->inel_ctr, queue);
 		if (queue) {
 			store_state(a.inel, qslot, s);
 			if (SC) store_score(a.inel, qslot, np, sc);
 		}
-		const bool survive = fin && keep && !to_inel;
+		const bool survive = keep && !to_inel;
 		const uint32_t oslot = wave_alloc(&C->surv_ctr, survive);
 		if (survive) {
 			store_state(out, oslot, s);
 			if (SC) store_score(out, oslot, np, sc);
 		}
-		if (fin)
-			have = false;
 	}
 	count_steps(C, my_steps);
+}
+
+// The plain variant fits 80 VGPRs without spilling: ask for 6 waves per SIMD
+// (the scored and multi-world variants would spill, so they are left free).
+#ifndef NBL_PLAIN_WAVES
+#define NBL_PLAIN_WAVES 6
+#endif
+template <bool SC, bool MULTI>
+__global__ __launch_bounds__(256) void k_step(StepArgs a)
+{
+	step_body<SC, MULTI>(a);
+}
+template <>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NBL_PLAIN_WAVES, 8)))
+void k_step<false, false>(StepArgs a)
+{
+	step_body<false, false>(a);
 }
 
 // The inelastic events queued by k_step (drivers/gpu/gpu_driver.inl:505-540):
@@ -483,13 +506,17 @@ __global__ __launch_bounds__(256) void k_inel(StepArgs a)
 		Particle se;
 		bool made = false;
 		step_event(W, s, rng, EV_INELASTIC, -1, se, made);
+		bool done = true;
 		if (place_secondary<SC>(a, C, out, se_limit, made, se, s, sc)) {
 			s.step++;
 		} else {
 			// no room anywhere: retry the event from the same draws later
 			s = pre;
 			s.flags |= FL_PEND_INEL;
+			done = false;
 		}
+		if ((SC || MULTI) && a.dep)
+			deposit(a, C, done && s.p.E != pre.p.E, s, pre.p.E);
 		const uint32_t oslot = wave_alloc(&C->surv_ctr, true);
 		store_state(out, oslot, s);
 		if (SC) store_score(out, oslot, np, sc);
@@ -645,8 +672,11 @@ struct nbl_driver {
 	double* sc_mem[4];      // score arrays of buf[0], buf[1], spill, inel
 	double* det_sc;
 	std::vector<Material> h_mats;   // host copy of d_mats
-	uint32_t blocks_per_cu = 8;     // persistent k_step grid
+	uint32_t blocks_per_cu = 32;    // k_step grid: blocks of 256 per compute unit
 	uint32_t target = 0;            // particles in flight to aim for
+	nbl_deposit* dep = nullptr;     // energy-deposit records (enabled when non-null)
+	uint64_t dep_cap = 0;
+	uint32_t dep_per_particle = 0;  // K + 1 when the buffer was sized
 	bool use_graph = false;         // replay iterations from captured graphs
 	uint64_t epoch = 0;             // bumped when a launch argument changes
 	struct Graph { uint32_t n; uint64_t epoch; hipGraph_t g; hipGraphExec_t exec; };
@@ -1002,6 +1032,7 @@ int nbl_driver_destroy(nbl_driver* d)
 	for (Material* p : d->world_mats)
 		hipFree(p);
 	hipFree(d->d_worlds);
+	hipFree(d->dep);
 	for (double* p : d->sc_mem)
 		hipFree(p);
 	hipFree(d->det_sc);
@@ -1078,6 +1109,9 @@ int nbl_driver_create(nbl_device* dev, uint32_t capacity, const nbl_geometry* ge
 	w0.mats = d->d_mats;
 	w0.nmats = n_mats;
 	d->h_worlds.push_back(w0);
+	if ((rc = dev_alloc(&d->d_worlds, 1))) return cleanup(rc);
+	if (hipMemcpy(d->d_worlds, &w0, sizeof(WorldDesc), hipMemcpyHostToDevice) != hipSuccess)
+		return cleanup(fail(NBL_ERR_DEVICE, "world table upload failed"));
 	*out = d;
 	return NBL_OK;
 }
@@ -1188,7 +1222,7 @@ namespace {
 // k_step + k_inel in the variant the driver needs (scores, several worlds).
 void launch_step_pair(nbl_driver* d, const StepArgs& a, uint32_t g)
 {
-	const bool multi = d->h_worlds.size() > 1;
+	const bool multi = d->h_worlds.size() > 1 || d->dep;
 	const dim3 grid(g), block(256);
 	if (d->map.np && multi) {
 		hipLaunchKernelGGL((k_step<true, true>), grid, block, 0, d->stream, a);
@@ -1220,6 +1254,8 @@ StepArgs step_args(const nbl_driver* d)
 	a.worlds = d->d_worlds;
 	a.det = d->det;
 	a.det_sc = d->det_sc;
+	a.dep = d->dep;
+	a.dep_cap = (uint32_t)std::min<uint64_t>(d->dep_cap, 0xffffffffu);
 	a.map = d->map;
 	a.det_cap = d->det_cap;
 	a.slots = d->slots;
@@ -1232,8 +1268,8 @@ StepArgs step_args(const nbl_driver* d)
 int enqueue_iteration(nbl_driver* d, const StepArgs& a)
 {
 	const uint32_t g_refill = blocks_for(d->capacity);
-	// persistent lanes: blocks_per_cu blocks of 256 per compute unit, never
-	// more than the particles in flight need
+	// grid-stride: blocks_per_cu x compute units blocks of 256 threads (a
+	// multiple of the resident blocks), never more than the capacity needs
 	const uint32_t g_step = std::min<uint32_t>(blocks_for(d->capacity), d->blocks_per_cu * (uint32_t)d->dev->n_cu);
 	hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(64), 0, d->stream, d->ctrl, d->slots, d->spill_cap);
 	hipLaunchKernelGGL(k_refill, dim3(g_refill), dim3(256), 0, d->stream, d->ctrl, d->buf[0],
@@ -1355,7 +1391,7 @@ int nbl_driver_autotune(nbl_driver* d, uint32_t iters_per_trial, nbl_tune_result
 	if ((rc = trial(best, rate))) { d->use_graph = graph; return rc; }
 	best_rate = rate;
 	const uint32_t Ks[] = { 4, 8, 16, 32 };
-	const uint32_t Bs[] = { 4, 5, 6, 8 };
+	const uint32_t Bs[] = { 4, 8, 16, 32 };
 	const uint32_t Ts[] = { std::max<uint32_t>(64, d->capacity / 4), std::max<uint32_t>(64, d->capacity / 2),
 		d->capacity };
 	// coordinate search: steps per launch, then grid, then particles in flight
@@ -1391,6 +1427,61 @@ int nbl_driver_autotune(nbl_driver* d, uint32_t iters_per_trial, nbl_tune_result
 	return NBL_OK;
 }
 
+int nbl_driver_set_deposits(nbl_driver* d, int enable)
+{
+	clear_error();
+	if (!d)
+		return fail(NBL_ERR_ARG, "null driver");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	HIP_OK(hipStreamSynchronize(d->stream));
+	hipFree(d->dep);
+	d->dep = nullptr;
+	d->dep_cap = 0;
+	d->dep_per_particle = 0;
+	d->epoch++;
+	if (!enable)
+		return NBL_OK;
+	// one record per step and one per inelastic event, for every particle
+	// the buffer can hold
+	d->dep_per_particle = d->K + 1;
+	d->dep_cap = (uint64_t)d->slots * d->dep_per_particle;
+	if (d->dep_cap > 0xffffffffull)
+		return fail(NBL_ERR_CAPACITY, "deposit buffer too large for this capacity and steps per launch");
+	if ((rc = dev_alloc(&d->dep, d->dep_cap)))
+		return rc;
+	const uint32_t zero = 0;
+	HIP_OK(hipMemcpy(&d->ctrl->dep_ctr, &zero, sizeof(zero), hipMemcpyHostToDevice));
+	return NBL_OK;
+}
+
+int nbl_driver_drain_deposits(nbl_driver* d, nbl_deposit* out, uint64_t cap, uint64_t* n)
+{
+	clear_error();
+	if (!d || !n || (cap && !out))
+		return fail(NBL_ERR_ARG, "invalid drain arguments");
+	if (!d->dep)
+		return fail(NBL_ERR_ARG, "deposits are not enabled on this driver");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	if ((rc = read_ctrl(d)))
+		return rc;
+	const uint64_t written = d->h_ctrl->dep_ctr;
+	if (written > d->dep_cap)
+		return fail(NBL_ERR_CAPACITY, "energy-deposit records were dropped: drain after every iteration");
+	if (written > cap) {
+		*n = written;
+		return fail(NBL_ERR_CAPACITY, "output array too small for " + std::to_string(written) + " records");
+	}
+	if (written)
+		HIP_OK(hipMemcpyAsync(out, d->dep, sizeof(nbl_deposit) * written, hipMemcpyDeviceToHost, d->stream));
+	const uint32_t zero = 0;
+	HIP_OK(hipMemcpyAsync(&d->ctrl->dep_ctr, &zero, sizeof(zero), hipMemcpyHostToDevice, d->stream));
+	HIP_OK(hipStreamSynchronize(d->stream));
+	*n = written;
+	return NBL_OK;
+}
+
 int nbl_driver_set_graph(nbl_driver* d, int enable)
 {
 	clear_error();
@@ -1411,6 +1502,8 @@ int nbl_driver_set_launch(nbl_driver* d, uint32_t steps_per_launch, uint32_t blo
 		return fail(NBL_ERR_ARG, "blocks_per_cu must be in [1, 32]");
 	if (target < 1 || target > d->capacity)
 		return fail(NBL_ERR_ARG, "target must be in [1, capacity]");
+	if (d->dep && steps_per_launch + 1 > d->dep_per_particle)
+		return fail(NBL_ERR_ARG, "deposits are enabled for fewer steps per launch: re-enable them after this change");
 	int rc = set_device(d->dev);
 	if (rc) return rc;
 	if (d->K != steps_per_launch || d->blocks_per_cu != blocks_per_cu)

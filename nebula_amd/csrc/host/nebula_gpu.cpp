@@ -26,6 +26,13 @@
 //  * Two options the reference fixes at compile time: --inelastic
 //    (penn = full_penn, the reference default, physics_config.h:21-24; or
 //    kieft = kieft_inelastic) and --steps-per-launch.
+//  * Energy deposits (the outputs of the reference's CPU-only
+//    nebula_cpu_edep, cpu_energydep.cpp:170-204): --deposit-filename writes
+//    one 28-byte record per scattering event that changes an electron's
+//    energy (float32 position, energy before, energy lost; int32 pixel[2]),
+//    --detect-filename moves the detected records off stdout.  Built as
+//    nebula_edep, the binary takes nebula_cpu_edep's defaults
+//    (detected.bin, deposits on stdout).
 #include "nebula.h"
 
 #include <algorithm>
@@ -132,9 +139,39 @@ void check(int rc, const char* what)
 
 const char* kParams = "[options] <geometry.tri> <primaries.pri> [material0.mat] .. [materialN.mat]";
 
-// Detected electrons -> 36-byte records on stdout, pixel looked up by tag.
+FILE* open_output(const std::string& name)
+{
+	if (name == "stdout")
+		return stdout;
+	FILE* f = std::fopen(name.c_str(), "wb");
+	if (!f)
+		throw Failure("cannot open " + name + " for writing");
+	return f;
+}
+
+// Energy deposits -> 28-byte records (cpu_energydep.cpp:192-199).
+void write_deposits(const std::vector<nbl_deposit>& dep, uint64_t n,
+	const std::vector<nbl_record36>& pri, FILE* out, std::vector<char>& buf)
+{
+	struct Rec { float pos[3], energy, deposit; int32_t pixel[2]; };
+	static_assert(sizeof(Rec) == 28, "record layout");
+	buf.resize((size_t)n * sizeof(Rec));
+	for (uint64_t i = 0; i < n; ++i) {
+		Rec r;
+		std::memcpy(r.pos, dep[i].pos, sizeof r.pos);
+		r.energy = dep[i].energy;
+		r.deposit = dep[i].deposit;
+		r.pixel[0] = pri[dep[i].tag].pixel[0];
+		r.pixel[1] = pri[dep[i].tag].pixel[1];
+		std::memcpy(buf.data() + (size_t)i * sizeof r, &r, sizeof r);
+	}
+	if (n && std::fwrite(buf.data(), sizeof(Rec), (size_t)n, out) != (size_t)n)
+		throw Failure("writing energy deposits failed");
+}
+
+// Detected electrons -> 36-byte records, pixel looked up by tag.
 void write_records(const std::vector<nbl_detected>& det, uint32_t n,
-	const std::vector<nbl_record36>& pri, std::vector<char>& buf)
+	const std::vector<nbl_record36>& pri, FILE* out, std::vector<char>& buf)
 {
 	buf.resize((size_t)n * sizeof(nbl_record36));
 	for (uint32_t i = 0; i < n; ++i) {
@@ -146,8 +183,8 @@ void write_records(const std::vector<nbl_detected>& det, uint32_t n,
 		r.pixel[1] = pri[det[i].tag].pixel[1];
 		std::memcpy(buf.data() + (size_t)i * sizeof r, &r, sizeof r);
 	}
-	if (n && std::fwrite(buf.data(), sizeof(nbl_record36), n, stdout) != n)
-		throw Failure("writing to stdout failed");
+	if (n && std::fwrite(buf.data(), sizeof(nbl_record36), n, out) != n)
+		throw Failure("writing detected electrons failed");
 }
 
 int run(int argc, char** argv)
@@ -169,6 +206,13 @@ int run(int argc, char** argv)
 	p.add("inelastic", "Inelastic model: penn (full_penn) or kieft (kieft_inelastic)", "penn");
 	p.add("steps-per-launch", "Steps each electron takes per kernel launch", 8);
 	p.add("device", "GPU ordinal", 0);
+#ifdef NBL_EDEP
+	p.add("detect-filename", "Filename for detected electrons", "detected.bin");
+	p.add("deposit-filename", "Filename for energy deposits", "stdout");
+#else
+	p.add("detect-filename", "Filename for detected electrons", "stdout");
+	p.add("deposit-filename", "Filename for energy deposits (empty: none)", "");
+#endif
 	p.parse(argc, argv);
 	const float energy_threshold = p.get<float>("energy-threshold");
 	const uint32_t capacity = p.get<uint32_t>("capacity");
@@ -179,6 +223,9 @@ int run(int argc, char** argv)
 	const std::string inel = p.get<std::string>("inelastic");
 	const uint32_t steps_per_launch = p.get<uint32_t>("steps-per-launch");
 	const int device = p.get<int>("device");
+	const std::string detect_name = p.opts.at("detect-filename").value;
+	const std::string deposit_name = p.opts.at("deposit-filename").value;
+	const bool deposits = !deposit_name.empty();
 
 	TimeLog timer;
 	const auto& pos = p.positional;
@@ -314,6 +361,24 @@ int run(int argc, char** argv)
 	nbl_driver* drv = nullptr;
 	check(nbl_driver_create(dev, capacity, geom, dmats.data(), (int32_t)dmats.size(), NBL_PHYS_DEFAULT,
 		energy_threshold, driver_seed, steps_per_launch, &drv), "creating the driver");
+	FILE* det_out = open_output(detect_name);
+	FILE* dep_out = deposits ? open_output(deposit_name) : nullptr;
+	std::vector<nbl_deposit> depbuf;
+	std::vector<char> dbuf;
+	// deposits are read after every iteration (the buffer holds one)
+	auto drain_deposits = [&]() {
+		uint64_t nd = 0;
+		const int rc = nbl_driver_drain_deposits(drv, nullptr, 0, &nd);
+		if (rc == NBL_OK)
+			return;
+		if (rc != NBL_ERR_CAPACITY || nd == 0)
+			check(rc, "reading energy deposits");
+		depbuf.resize(nd);
+		check(nbl_driver_drain_deposits(drv, depbuf.data(), nd, &nd), "reading energy deposits");
+		write_deposits(depbuf, nd, pri, dep_out, dbuf);
+	};
+	if (deposits)
+		check(nbl_driver_set_deposits(drv, 1), "enabling energy deposits");
 
 	// ----------------------------------------------------------- prescan
 	// Prescan primaries are real work; their detections are written first.
@@ -329,6 +394,8 @@ int run(int argc, char** argv)
 	int stalled = 0;
 	while (stats.back().first > 0) {
 		check(nbl_driver_iterate(drv, 1), "iterate");
+		if (deposits)
+			drain_deposits();
 		uint32_t running = 0, detected = 0;
 		uint64_t queued = 0, steps = 0;
 		check(nbl_driver_counts(drv, &running, &detected, &queued), "counts");
@@ -377,8 +444,10 @@ int run(int argc, char** argv)
 		check(nbl_driver_steps(drv, &steps), "steps");
 		const bool done = running == 0 && queued == 0;
 		if (!done)
-			check(nbl_driver_iterate(drv, frame_size), "iterate");
-		write_records(det, got, pri, obuf);   // overlaps the frame just launched
+			check(nbl_driver_iterate(drv, deposits ? 1 : frame_size), "iterate");
+		write_records(det, got, pri, det_out, obuf);   // overlaps the frame just launched
+		if (deposits && !done)
+			drain_deposits();
 		const auto now = std::chrono::steady_clock::now();
 		if (done || now - last_print >= std::chrono::seconds(1)) {
 			std::clog << " \rProgress " << std::fixed << std::setprecision(2)
@@ -393,7 +462,14 @@ int run(int argc, char** argv)
 			throw Failure("no progress: particle buffer and spill stack full of particles waiting "
 				"for secondary-electron slots; raise --capacity");
 	}
-	std::fflush(stdout);
+	std::fflush(det_out);
+	if (det_out != stdout)
+		std::fclose(det_out);
+	if (dep_out) {
+		std::fflush(dep_out);
+		if (dep_out != stdout)
+			std::fclose(dep_out);
+	}
 	timer.stop("Simulation");
 	const int rc_sync = nbl_device_synchronize(dev);
 	std::clog << (rc_sync == NBL_OK ? "\nSimulation successful!\n\n" : "\nSimulation ended with a GPU error\n\n");

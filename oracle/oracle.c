@@ -1723,9 +1723,11 @@ int orc_step(const orc_world* w, int semantics, orc_state* s, orc_rng* rng,
 /* --------------------------------------------------------------- drivers -- */
 typedef struct { orc_state s; int alive; } ref_slot;
 
-int64_t orc_run_reference(const orc_world* w, const orc_particle* primaries,
-	size_t n, uint32_t seed, orc_detected* out, size_t cap, uint64_t* n_steps)
+int64_t orc_run_reference_deposits(const orc_world* w, const orc_particle* primaries,
+	size_t n, uint32_t seed, orc_detected* out, size_t cap, orc_deposit* dep, size_t dep_cap,
+	uint64_t* n_dep, uint64_t* n_steps)
 {
+	uint64_t nd = 0;
 	orc_rng rng;
 	orc_rng_init_mt(&rng, seed);
 	size_t vcap = 1024, vn = 0;
@@ -1751,7 +1753,19 @@ int64_t orc_run_reference(const orc_world* w, const orc_particle* primaries,
 				orc_state se;
 				int made, ev;
 				++steps;
+				const float e0 = v[idx].s.p.E;
 				status[idx] = orc_step(w, 0, &v[idx].s, &rng, &se, &made, &ev);
+				/* energydeposit_cpu_driver.h:79-89: scattering events only */
+				if (dep && (ev == ORC_EV_ELASTIC || ev == ORC_EV_INELASTIC) && v[idx].s.p.E != e0) {
+					if (nd < dep_cap) {
+						dep[nd].pos = v[idx].s.p.pos;
+						dep[nd].energy = e0;
+						dep[nd].deposit = e0 - v[idx].s.p.E;
+						dep[nd].tag = v[idx].s.tag;
+						dep[nd].sid = v[idx].s.sid;
+					}
+					++nd;
+				}
 				if (made) {
 					if (vn == vcap) {
 						vcap *= 2;
@@ -1778,7 +1792,15 @@ int64_t orc_run_reference(const orc_world* w, const orc_particle* primaries,
 	free(v);
 	free(status);
 	if (n_steps) *n_steps = steps;
+	if (n_dep) *n_dep = nd;
+	if (nd > dep_cap) return -1;
 	return nout;
+}
+
+int64_t orc_run_reference(const orc_world* w, const orc_particle* primaries,
+	size_t n, uint32_t seed, orc_detected* out, size_t cap, uint64_t* n_steps)
+{
+	return orc_run_reference_deposits(w, primaries, n, seed, out, cap, NULL, 0, NULL, n_steps);
 }
 
 typedef struct {
@@ -1791,6 +1813,9 @@ typedef struct {
 	atomic_size_t next;
 	orc_detected* out;
 	double* out_scores;
+	orc_deposit* dep;             /* NULL: no deposit records */
+	size_t dep_cap;
+	atomic_ullong ndep;
 	size_t cap;
 	atomic_llong nout;
 	atomic_ullong steps;
@@ -1830,7 +1855,19 @@ static void* philox_worker(void* arg)
 					int made, ev;
 					orc_rng_set_stream(&rng, s.sid, s.step);
 					++steps;
+					const float e0 = s.p.E;
 					const int st = orc_step_scored(job->w, 1, &s, &rng, &se, &made, &ev, map, cur.score);
+					if (job->dep && (ev == ORC_EV_ELASTIC || ev == ORC_EV_INELASTIC) && s.p.E != e0) {
+						const unsigned long long slot = atomic_fetch_add(&job->ndep, 1);
+						if (slot < job->dep_cap) {
+							orc_deposit* r = &job->dep[slot];
+							r->pos = s.p.pos;
+							r->energy = e0;
+							r->deposit = e0 - s.p.E;
+							r->tag = s.tag;
+							r->sid = s.sid;
+						}
+					}
 					s.step++;
 					if (made) {
 						if (sn == scap) {
@@ -1890,6 +1927,34 @@ int64_t orc_run_philox_scored(const orc_world* w, const orc_particle* primaries,
 	free(th);
 	if (n_steps) *n_steps = atomic_load(&job.steps);
 	if (atomic_load(&job.overflow)) return -1;
+	return (int64_t)atomic_load(&job.nout);
+}
+
+int64_t orc_run_philox_deposits(const orc_world* w, const orc_particle* primaries,
+	size_t n, uint64_t seed, uint64_t sid_base, uint32_t tag_base, int threads,
+	orc_detected* out, size_t cap, orc_deposit* dep, size_t dep_cap, uint64_t* n_dep,
+	uint64_t* n_steps)
+{
+	philox_job job;
+	memset(&job, 0, sizeof(job));
+	job.w = w; job.primaries = primaries; job.n = n;
+	job.seed = seed; job.sid_base = sid_base; job.tag_base = tag_base;
+	job.out = out; job.cap = cap; job.dep = dep; job.dep_cap = dep_cap;
+	atomic_init(&job.next, 0);
+	atomic_init(&job.nout, 0);
+	atomic_init(&job.ndep, 0);
+	atomic_init(&job.steps, 0);
+	atomic_init(&job.overflow, 0);
+	if (threads < 1) threads = 1;
+	pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
+	for (int t = 0; t < threads; ++t)
+		pthread_create(&th[t], NULL, philox_worker, &job);
+	for (int t = 0; t < threads; ++t)
+		pthread_join(th[t], NULL);
+	free(th);
+	if (n_steps) *n_steps = atomic_load(&job.steps);
+	if (n_dep) *n_dep = atomic_load(&job.ndep);
+	if (atomic_load(&job.overflow) || atomic_load(&job.ndep) > dep_cap) return -1;
 	return (int64_t)atomic_load(&job.nout);
 }
 

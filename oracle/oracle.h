@@ -315,6 +315,31 @@ int64_t orc_run_philox(const orc_world* w, const orc_particle* primaries,
 	size_t n, uint64_t seed, uint64_t sid_base, uint32_t tag_base, int threads,
 	orc_detected* out, size_t cap, uint64_t* n_steps);
 
+/* Energy-deposit record (drivers/cpu/energydeposit_cpu_driver.h:55-66):
+ * position after a scattering event that changed the energy, energy
+ * before, energy lost, primary tag, stream id. */
+typedef struct {
+	orc_vec3 pos;
+	float energy, deposit;
+	uint32_t tag;
+	uint64_t sid;
+} orc_deposit;
+
+/* orc_run_reference that also writes energy-deposit records in the order of
+ * energydeposit_cpu_driver::simulate_to_end (energydeposit_cpu_driver.h:
+ * 75-91); *n_dep receives their number (-1 is returned if above dep_cap). */
+int64_t orc_run_reference_deposits(const orc_world* w, const orc_particle* primaries,
+	size_t n, uint32_t seed, orc_detected* out, size_t cap, orc_deposit* dep, size_t dep_cap,
+	uint64_t* n_dep, uint64_t* n_steps);
+
+/* orc_run_philox that also writes the energy-deposit records (elastic and
+ * inelastic events whose energy change is nonzero) into dep[dep_cap];
+ * *n_dep receives their number (may exceed dep_cap: then -1 is returned). */
+int64_t orc_run_philox_deposits(const orc_world* w, const orc_particle* primaries,
+	size_t n, uint64_t seed, uint64_t sid_base, uint32_t tag_base, int threads,
+	orc_detected* out, size_t cap, orc_deposit* dep, size_t dep_cap, uint64_t* n_dep,
+	uint64_t* n_steps);
+
 /* orc_run_philox with scores: out_scores[i * map->n_params + k] is the
  * score of detected record i for parameter k (secondaries inherit their
  * parent's score at creation). */

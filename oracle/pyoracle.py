@@ -116,6 +116,8 @@ class RawMaterial(C.Structure):
                 ("n_outer_shells", C.c_int), ("outer_shells_unit", C.c_double)]
 
 
+DEP_DTYPE = np.dtype([("pos", "<f4", 3), ("energy", "<f4"), ("deposit", "<f4"), ("tag", "<u4"),
+                      ("sid", "<u8")])
 DET_DTYPE = np.dtype([("pos", "<f4", 3), ("dir", "<f4", 3), ("E", "<f4"), ("tag", "<u4"),
                       ("sid", "<u8")])
 STATE_DTYPE = np.dtype([("pos", "<f4", 3), ("dir", "<f4", 3), ("E", "<f4"), ("mat", "<i4"),
@@ -150,6 +152,13 @@ def load(kind: str = "pf"):
         L.orc_run_philox.argtypes = [C.POINTER(World), vp, C.c_size_t, C.c_uint64, C.c_uint64, C.c_uint32,
                                      C.c_int, vp, C.c_size_t, C.POINTER(C.c_uint64)]
         L.orc_run_philox.restype = C.c_int64
+        L.orc_run_reference_deposits.argtypes = [C.POINTER(World), vp, C.c_size_t, C.c_uint32, vp, C.c_size_t,
+                                                 vp, C.c_size_t, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.orc_run_reference_deposits.restype = C.c_int64
+        L.orc_run_philox_deposits.argtypes = [C.POINTER(World), vp, C.c_size_t, C.c_uint64, C.c_uint64,
+                                              C.c_uint32, C.c_int, vp, C.c_size_t, vp, C.c_size_t,
+                                              C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.orc_run_philox_deposits.restype = C.c_int64
         L.orc_run_philox_scored.argtypes = [C.POINTER(World), vp, C.c_size_t, C.c_uint64, C.c_uint64,
                                             C.c_uint32, C.c_int, C.POINTER(ScoreMap), vp, vp, C.c_size_t,
                                             C.POINTER(C.c_uint64)]
@@ -373,6 +382,33 @@ class OracleWorld:
             m.in_scale = in_scale
         if barrier is not None:
             m.barrier = barrier
+
+    def run_reference_deposits(self, primaries: np.ndarray, seed: int, cap: int, dep_cap: int):
+        """run_reference plus energy-deposit records in the reference
+        energydeposit_cpu_driver's order."""
+        p = _particles(primaries)
+        out = np.zeros(cap, DET_DTYPE)
+        dep = np.zeros(dep_cap, DEP_DTYPE)
+        nd, steps = C.c_uint64(), C.c_uint64()
+        n = self.L.orc_run_reference_deposits(C.byref(self.w), _p(p), len(p), seed, _p(out), cap, _p(dep), dep_cap,
+                                              C.byref(nd), C.byref(steps))
+        if n < 0:
+            raise ValueError("output capacity exceeded")
+        return out[:n], dep[:nd.value], steps.value
+
+    def run_philox_deposits(self, primaries: np.ndarray, seed: int, cap: int, dep_cap: int, threads=1,
+                            sid_base=0, tag_base=0):
+        """run_philox plus the energy-deposit records of every scattering
+        event that changed an electron's energy."""
+        p = _particles(primaries)
+        out = np.zeros(cap, DET_DTYPE)
+        dep = np.zeros(dep_cap, DEP_DTYPE)
+        nd, steps = C.c_uint64(), C.c_uint64()
+        n = self.L.orc_run_philox_deposits(C.byref(self.w), _p(p), len(p), seed, sid_base, tag_base, threads,
+                                           _p(out), cap, _p(dep), dep_cap, C.byref(nd), C.byref(steps))
+        if n < 0:
+            raise ValueError("output capacity exceeded")
+        return out[:n], dep[:nd.value], steps.value
 
     def run_philox_scored(self, primaries: np.ndarray, seed: int, cap: int, params, threads=1,
                           sid_base=0, tag_base=0):

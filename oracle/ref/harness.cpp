@@ -10,6 +10,10 @@
 //             its own mt19937(seed + i)
 //   simulate  <tris.bin> <penn|kieft> <seed> <pri.bin> <out.bin> <mat>...
 //             simple_cpu_driver over all primaries, one thread
+//   edep      <tris.bin> <penn|kieft> <seed> <pri.bin> <out.bin> <mat>...
+//             energydeposit_cpu_driver over all primaries, one thread:
+//             deposit records (float pos[3], E, dE, uint32 tag) in callback
+//             order, int64 count; then detected records as for simulate
 //   pri       <in.pri> <prescan_size> <sort 0|1> <out.bin>
 //             load_pri_file, optional sort_pri_file, prescan_shuffle, in the
 //             order of gpu_main.cu:364-367; out = the 36-byte records
@@ -22,6 +26,7 @@
 #include "physics_config.h"
 #include "core/material.h"
 #include "drivers/cpu/simple_cpu_driver.h"
+#include "drivers/cpu/energydeposit_cpu_driver.h"
 #include "geometry/octree.h"
 #include "io/load_pri_file.h"
 #undef private
@@ -289,6 +294,43 @@ template <typename L> static int cmd_simulate(char** a, int nmat)
 	return 0;
 }
 
+template <typename L> static int cmd_edep(char** a, int nmat)
+{
+	auto tris = read_tris(a[0]);
+	const uint32_t seed = (uint32_t)std::stoul(a[2]);
+	std::vector<material<L>> mats;
+	for (int i = 0; i < nmat; ++i)
+		mats.push_back(material<L>(nbl::hdf5_file(a[5 + i])));
+	geometry_t g = geometry_t::create(tris);
+	std::vector<particle> prim;
+	std::vector<int2> pix;
+	std::tie(prim, pix) = nbl::load_pri_file(a[3], g.AABB_min(), g.AABB_max(), 1e9f);
+	intersect_t inter;
+	nbl::drivers::energydeposit_cpu_driver<L, intersect_t, geometry_t> d(g, inter, mats, 0, seed);
+	struct Dep { float pos[3], E, dE; uint32_t tag; };
+	std::vector<Dep> deps;
+	std::vector<std::pair<particle, uint32_t>> det;
+	for (size_t i = 0; i < prim.size(); ++i) {
+		uint32_t tag = (uint32_t)i;
+		d.push(&prim[i], &tag, 1);
+		d.simulate_to_end([&](particle const& before, particle const& after, uint32_t t) {
+			deps.push_back({ { after.pos.x, after.pos.y, after.pos.z }, before.kin_energy,
+				before.kin_energy - after.kin_energy, t });
+		});
+		d.flush_detected([&](particle p, uint32_t t) { det.push_back({ p, t }); });
+	}
+	Out o(a[4]);
+	for (const Dep& r : deps)
+		o.put(r);
+	o.put((int64_t)deps.size());
+	for (const auto& r : det) {
+		o.put(r.first);
+		o.put(r.second);
+	}
+	o.put((int64_t)det.size());
+	return 0;
+}
+
 static int cmd_pri(char** a)
 {
 	const float big = std::numeric_limits<float>::max();
@@ -325,6 +367,7 @@ int main(int argc, char** argv)
 		if (c == "probe") return penn ? cmd_probe<penn_list>(a) : cmd_probe<kieft_list>(a);
 		if (c == "step") return penn ? cmd_step<penn_list>(a, na - 5) : cmd_step<kieft_list>(a, na - 5);
 		if (c == "simulate") return penn ? cmd_simulate<penn_list>(a, na - 5) : cmd_simulate<kieft_list>(a, na - 5);
+		if (c == "edep") return penn ? cmd_edep<penn_list>(a, na - 5) : cmd_edep<kieft_list>(a, na - 5);
 	} catch (const std::exception& e) {
 		std::fprintf(stderr, "harness error: %s\n", e.what());
 		return 1;

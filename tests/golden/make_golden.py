@@ -221,6 +221,24 @@ def main():
         fx[f"sim_{model}_pri"] = pri
         fx[f"sim_{model}_det"] = det.copy()
         print(f"simulate {model}: {len(pri)} primaries, {n} detected")
+        # energy deposits (energydeposit_cpu_driver) for the first 8 primaries
+        pf32 = os.path.join(tmp, "pri32.bin")
+        W.write_pri(pf32, pri[:8])
+        ef = os.path.join(tmp, "edep.out")
+        run("edep", tf, model, 78, pf32, ef, *mats)
+        b = np.fromfile(ef, np.uint8)
+        dep_dt = np.dtype([("pos", "<f4", 3), ("E", "<f4"), ("dE", "<f4"), ("tag", "<u4")])
+        # layout: deposits, int64 count, detected records, int64 count
+        n_det = int(b[-8:].view("<i8")[0])
+        det_dt = np.dtype([("p", PART), ("tag", "<u4")])
+        det_bytes = n_det * det_dt.itemsize
+        head = b[: len(b) - 8 - det_bytes]
+        nd = int(head[-8:].view("<i8")[0])
+        deps = head[:-8].view(dep_dt)
+        assert len(deps) == nd
+        fx[f"edep_{model}_dep"] = deps.copy()
+        fx[f"edep_{model}_det"] = b[len(head): len(b) - 8].view(det_dt).copy()
+        print(f"edep {model}: 8 primaries, {nd} deposits, {n_det} detected")
     # ------------------------------------- primary order: sort + prescan shuffle
     # 3000 primaries on a coarse lattice (many equal Morton codes, so the
     # unstable std::sort order matters) at one height (a zero-extent axis).

@@ -71,3 +71,39 @@ def test_cli_records_equal_oracle(tmp_path, capacity, prescan, sort):
     assert len(got) > 100
     assert len(got) == len(exp)
     assert np.array_equal(_as_sorted_bytes(got), _as_sorted_bytes(exp))
+
+
+def test_cli_energy_deposits_equal_oracle(tmp_path):
+    """nebula_edep (nebula_cpu_edep's outputs): 28-byte deposit records and
+    36-byte detected records equal the oracle's as multisets."""
+    mesh = W.grating_geometry(dy=3.2)
+    pri = W.raster_scan(3, 3, 6, 300.0, -48.0, 48.0, -48.0, 48.0, 37.0, seed=9)
+    W.write_tri(str(tmp_path / "g.tri"), mesh)
+    W.write_pri(str(tmp_path / "s.pri"), pri)
+    A.write_mat_file(str(tmp_path / "si.mat"), material("si"))
+    A.write_mat_file(str(tmp_path / "pmma.mat"), material("pmma"))
+    seed, prescan = 11, 20
+    cli = os.path.join(_lib.LIB_DIR, "nebula_edep")
+    print("nebula_edep", flush=True)
+    r = subprocess.run([cli, "--capacity", "4096", "--prescan-size", str(prescan), "--seed", str(seed),
+                        "--detect-filename", str(tmp_path / "det.bin"),
+                        str(tmp_path / "g.tri"), str(tmp_path / "s.pri"),
+                        str(tmp_path / "si.mat"), str(tmp_path / "pmma.mat")],
+                       capture_output=True, timeout=100)
+    assert r.returncode == 0, r.stderr.decode()[-2000:]
+    dep_dt = np.dtype([("pos", "<f4", 3), ("E", "<f4"), ("dE", "<f4"), ("px", "<i4"), ("py", "<i4")])
+    got_dep = np.frombuffer(r.stdout, dep_dt)
+    got_det = np.fromfile(tmp_path / "det.bin", W.PRI_DTYPE)
+    rec = A.prescan_shuffle(pri, prescan)
+    ow = O.OracleWorld(A.mesh_to_triangles(mesh), [material("si"), material("pmma")])
+    det, dep, _ = ow.run_philox_deposits(rec, _mt_first(seed), 100000, 2000000, threads=8)
+    exp_dep = np.zeros(len(dep), dep_dt)
+    exp_dep["pos"], exp_dep["E"], exp_dep["dE"] = dep["pos"], dep["energy"], dep["deposit"]
+    exp_dep["px"], exp_dep["py"] = rec["px"][dep["tag"]], rec["py"][dep["tag"]]
+    exp_det = np.zeros(len(det), W.PRI_DTYPE)
+    exp_det["pos"], exp_det["dir"], exp_det["E"] = det["pos"], det["dir"], det["E"]
+    exp_det["px"], exp_det["py"] = rec["px"][det["tag"]], rec["py"][det["tag"]]
+    print(f"deposits: cli={len(got_dep)} oracle={len(exp_dep)}; detected: {len(got_det)}")
+    assert len(got_dep) > 1000 and len(got_dep) == len(exp_dep)
+    assert np.array_equal(_as_sorted_bytes(got_dep), _as_sorted_bytes(exp_dep))
+    assert np.array_equal(_as_sorted_bytes(got_det), _as_sorted_bytes(exp_det))

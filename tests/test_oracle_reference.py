@@ -194,3 +194,25 @@ def test_whole_cascades_match_reference(model):
     assert np.array_equal(det["tag"], ref["tag"])
     for k in ("pos", "dir", "E"):
         assert np.array_equal(bits(det[k]), bits(ref["p"][k])), k
+
+
+@pytest.mark.parametrize("model", ["penn", "kieft"])
+def test_energy_deposits_match_reference(model):
+    """energydeposit_cpu_driver (drivers/cpu/energydeposit_cpu_driver.h:
+    75-91) over the first 8 primaries with one mt19937 stream: every
+    deposit record (position after the event, energy before, energy lost,
+    tag) in callback order, and the detected electrons, bit for bit."""
+    ow = _world(model)
+    pri = FX[f"sim_{model}_pri"][:8]
+    ref = FX[f"edep_{model}_dep"]
+    det, dep, _ = ow.run_reference_deposits(pri, 78, 10000, 200000)
+    assert len(dep) == len(ref) > 1000
+    assert np.array_equal(dep["tag"], ref["tag"])
+    assert np.array_equal(bits(dep["pos"]), bits(ref["pos"]))
+    assert np.array_equal(bits(dep["energy"]), bits(ref["E"]))
+    assert np.array_equal(bits(dep["deposit"]), bits(ref["dE"]))
+    rdet = FX[f"edep_{model}_det"]
+    assert len(det) == len(rdet)
+    assert np.array_equal(det["tag"], rdet["tag"])
+    assert np.array_equal(bits(det["E"]), bits(rdet["p"]["E"]))
+
