@@ -31,14 +31,25 @@ using namespace nbl::dev;
 	return fail(NBL_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); } while (0)
 
 // ------------------------------------------------------------- buffers --
+// One allocation per buffer of n entries: sid (8 bytes x n), then twelve
+// 4-byte fields x n (px py pz dx dy dz E mat last_tri tag step flags).  A
+// buffer is one base pointer and its size, so a kernel holds four buffers
+// in a few scalar registers instead of 13 pointers each.
 struct Bufs {
-	float *px, *py, *pz, *dx, *dy, *dz, *E;
-	int *mat, *last_tri;
-	uint32_t *tag, *step, *flags;
-	uint64_t* sid;
+	char* base;
 	double* sc;             // scores, [parameter][entry], or null
-	uint32_t sc_stride;     // entries of this buffer
+	uint32_t sc_stride;     // entries of this buffer (n)
 };
+enum { F_PX, F_PY, F_PZ, F_DX, F_DY, F_DZ, F_E, F_MAT, F_LAST, F_TAG, F_STEP, F_FLAGS, N_FIELDS };
+
+__device__ __forceinline__ float* fld_f(const Bufs& b, int f)
+{
+	return (float*)(b.base + 8ull * b.sc_stride) + (size_t)f * b.sc_stride;
+}
+__device__ __forceinline__ uint32_t* fld_u(const Bufs& b, int f)
+{
+	return (uint32_t*)(b.base + 8ull * b.sc_stride) + (size_t)f * b.sc_stride;
+}
 
 // Likelihood-ratio scores carried by a particle (FP64 accumulation).
 constexpr int MAX_PARAMS = 8;
@@ -92,29 +103,29 @@ struct StepArgs {
 __device__ __forceinline__ State load_state(const Bufs& b, uint32_t i)
 {
 	State s;
-	s.p.pos = { b.px[i], b.py[i], b.pz[i] };
-	s.p.dir = { b.dx[i], b.dy[i], b.dz[i] };
-	s.p.E = b.E[i];
-	s.mat = b.mat[i];
-	s.last_tri = b.last_tri[i];
-	s.tag = b.tag[i];
-	s.step = b.step[i];
-	s.flags = b.flags[i];
-	s.sid = b.sid[i];
+	s.p.pos = { fld_f(b, F_PX)[i], fld_f(b, F_PY)[i], fld_f(b, F_PZ)[i] };
+	s.p.dir = { fld_f(b, F_DX)[i], fld_f(b, F_DY)[i], fld_f(b, F_DZ)[i] };
+	s.p.E = fld_f(b, F_E)[i];
+	s.mat = (int)fld_u(b, F_MAT)[i];
+	s.last_tri = (int)fld_u(b, F_LAST)[i];
+	s.tag = fld_u(b, F_TAG)[i];
+	s.step = fld_u(b, F_STEP)[i];
+	s.flags = fld_u(b, F_FLAGS)[i];
+	s.sid = ((const uint64_t*)b.base)[i];
 	return s;
 }
 
 __device__ __forceinline__ void store_state(const Bufs& b, uint32_t i, const State& s)
 {
-	b.px[i] = s.p.pos.x; b.py[i] = s.p.pos.y; b.pz[i] = s.p.pos.z;
-	b.dx[i] = s.p.dir.x; b.dy[i] = s.p.dir.y; b.dz[i] = s.p.dir.z;
-	b.E[i] = s.p.E;
-	b.mat[i] = s.mat;
-	b.last_tri[i] = s.last_tri;
-	b.tag[i] = s.tag;
-	b.step[i] = s.step;
-	b.flags[i] = s.flags;
-	b.sid[i] = s.sid;
+	fld_f(b, F_PX)[i] = s.p.pos.x; fld_f(b, F_PY)[i] = s.p.pos.y; fld_f(b, F_PZ)[i] = s.p.pos.z;
+	fld_f(b, F_DX)[i] = s.p.dir.x; fld_f(b, F_DY)[i] = s.p.dir.y; fld_f(b, F_DZ)[i] = s.p.dir.z;
+	fld_f(b, F_E)[i] = s.p.E;
+	fld_u(b, F_MAT)[i] = (uint32_t)s.mat;
+	fld_u(b, F_LAST)[i] = (uint32_t)s.last_tri;
+	fld_u(b, F_TAG)[i] = s.tag;
+	fld_u(b, F_STEP)[i] = s.step;
+	fld_u(b, F_FLAGS)[i] = s.flags;
+	((uint64_t*)b.base)[i] = s.sid;
 }
 
 __device__ __forceinline__ void load_score(const Bufs& b, uint32_t i, int np, Score& sc)
@@ -396,17 +407,18 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 					if (ev == EV_INELASTIC) {
 						to_inel = true;
 					} else {
-						Particle unused;
-						bool made = false;
 						if (SC) {
 							float g = 0.0f;
 							int gmi = -1, gmo = -1;
-							oc = step_event(W, s, rng, ev, hit, unused, made, &g, &gmi, &gmo);
+							const float e0 = s.p.E;
+							oc = step_event_elastic_or_boundary(W, s, rng, ev, hit, &g, &gmi, &gmo);
 							if (ev == EV_INTERSECT)
 								score_barrier(sc, a.map, gmi, gmo, g);
+							if (DEP_OK && a.dep)
+								deposit(a, C, ev == EV_ELASTIC && s.p.E != e0, s, e0);
 						} else {
 							const float e0 = s.p.E;
-							oc = step_event(W, s, rng, ev, hit, unused, made);
+							oc = step_event_elastic_or_boundary(W, s, rng, ev, hit);
 							if (DEP_OK && a.dep)
 								deposit(a, C, ev == EV_ELASTIC && s.p.E != e0, s, e0);
 						}
@@ -738,20 +750,12 @@ uint32_t blocks_for(uint64_t n, uint32_t bs = 256)
 int alloc_bufs(Bufs& b, void*& mem, uint32_t slots)
 {
 	const size_t n = slots;
-	const size_t bytes = n * (7 * 4 + 2 * 4 + 3 * 4 + 8) + 256;
+	const size_t bytes = n * (8 + 4 * N_FIELDS) + 256;
 	char* p = nullptr;
 	if (hipMalloc((void**)&p, bytes) != hipSuccess)
 		return fail(NBL_ERR_ALLOC, "hipMalloc of the particle buffers failed");
 	mem = p;
-	// sid first (8-byte aligned), then the 4-byte fields
-	b.sid = (uint64_t*)p; p += 8 * n;
-	float** f[7] = { &b.px, &b.py, &b.pz, &b.dx, &b.dy, &b.dz, &b.E };
-	for (int k = 0; k < 7; ++k) { *f[k] = (float*)p; p += 4 * n; }
-	b.mat = (int*)p; p += 4 * n;
-	b.last_tri = (int*)p; p += 4 * n;
-	b.tag = (uint32_t*)p; p += 4 * n;
-	b.step = (uint32_t*)p; p += 4 * n;
-	b.flags = (uint32_t*)p; p += 4 * n;
+	b.base = p;
 	b.sc = nullptr;
 	b.sc_stride = slots;
 	return NBL_OK;

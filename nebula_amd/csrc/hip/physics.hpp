@@ -1008,4 +1008,22 @@ __device__ inline int step_event(const World& W, State& s, Rng& rng, int event, 
 	return ALIVE;
 }
 
+// The events of a step other than inelastic scattering: boundary crossing
+// and elastic scattering (and nothing for EV_NONE).  The tracking kernel
+// sends inelastic events to their own kernel, so its code holds neither the
+// inelastic models nor their register pressure.
+__device__ inline int step_event_elastic_or_boundary(const World& W, State& s, Rng& rng, int event,
+	int hit, float* g = nullptr, int* g_mi = nullptr, int* g_mo = nullptr)
+{
+	if (event == EV_INTERSECT) {
+		const Tri tri = load_tri(W.g.tris, hit);
+		return boundary_execute(W.mats, W.flags, tri, s.p, s.mat, rng, g, g_mi, g_mo);
+	}
+	if (event == EV_ELASTIC) {
+		s.last_tri = -1;
+		elastic_execute(W.mats[s.mat], W.flags, s.p, rng);
+	}
+	return ALIVE;
+}
+
 }} // namespace nbl::dev
