@@ -470,10 +470,12 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	count_steps(C, my_steps);
 }
 
-// The plain variant fits 80 VGPRs without spilling: ask for 6 waves per SIMD
+// The plain variant runs at 7 waves per SIMD (the LDS path cache allows 7
+// blocks per compute unit; 72 VGPRs and a 20-byte spill measured faster
+// than 6 waves at 80 VGPRs)
 // (the scored and multi-world variants would spill, so they are left free).
 #ifndef NBL_PLAIN_WAVES
-#define NBL_PLAIN_WAVES 6
+#define NBL_PLAIN_WAVES 7
 #endif
 template <bool SC, bool MULTI>
 __global__ __launch_bounds__(256) void k_step(StepArgs a)
@@ -644,7 +646,6 @@ struct nbl_geometry {
 	nbl_device* dev;
 	int* od;
 	float4* tris;
-	float4* tside;
 	Geometry g;
 	int max_material;   // largest physical material index named by a triangle
 };
@@ -887,22 +888,16 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 			}
 		}
 	}
-	for (int64_t i = 0; i < ntri; ++i)
-		for (int32_t m : { tris[i].material_in, tris[i].material_out })
-			if (m < -32768 || m > 32767)
-				return fail(NBL_ERR_FORMAT, "material index " + std::to_string(m) + " out of the int16 range");
 	int rc = set_device(dev);
 	if (rc) return rc;
 	auto* g = new nbl_geometry();
 	g->dev = dev;
-	if ((rc = dev_alloc(&g->od, (size_t)n_od)) || (rc = dev_alloc(&g->tris, (size_t)ntri * 3)) ||
-		(rc = dev_alloc(&g->tside, (size_t)ntri))) {
+	if ((rc = dev_alloc(&g->od, (size_t)n_od)) || (rc = dev_alloc(&g->tris, (size_t)ntri * 3))) {
 		hipFree(g->od);
-		hipFree(g->tris);
 		delete g;
 		return rc;
 	}
-	std::vector<float4> packed((size_t)ntri * 3), side((size_t)ntri);
+	std::vector<float4> packed((size_t)ntri * 3);
 	for (int64_t i = 0; i < ntri; ++i) {
 		const nbl_triangle44& t = tris[i];
 		float ib, ob;
@@ -911,22 +906,11 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 		packed[3 * i] = make_float4(t.r0[0], t.r0[1], t.r0[2], t.e1[0]);
 		packed[3 * i + 1] = make_float4(t.e1[1], t.e1[2], t.e2[0], t.e2[1]);
 		packed[3 * i + 2] = make_float4(t.e2[2], ib, ob, 0.0f);
-		// e1 x e2 in the walk's expression order (no contraction)
-		const float nx = t.e1[1] * t.e2[2] - t.e1[2] * t.e2[1];
-		const float ny = t.e1[2] * t.e2[0] - t.e1[0] * t.e2[2];
-		const float nz = t.e1[0] * t.e2[1] - t.e1[1] * t.e2[0];
-		const uint32_t mp = ((uint32_t)(uint16_t)(int16_t)t.material_in) |
-			((uint32_t)(uint16_t)(int16_t)t.material_out << 16);
-		float mf;
-		std::memcpy(&mf, &mp, 4);
-		side[i] = make_float4(nx, ny, nz, mf);
 	}
 	HIP_OK(hipMemcpy(g->od, od, sizeof(int32_t) * n_od, hipMemcpyHostToDevice));
 	HIP_OK(hipMemcpy(g->tris, packed.data(), sizeof(float4) * packed.size(), hipMemcpyHostToDevice));
-	HIP_OK(hipMemcpy(g->tside, side.data(), sizeof(float4) * side.size(), hipMemcpyHostToDevice));
 	g->g.od = g->od;
 	g->g.tris = g->tris;
-	g->g.tside = g->tside;
 	g->g.center = { center[0], center[1], center[2] };
 	g->g.halfsize = { halfsize[0], halfsize[1], halfsize[2] };
 	// octree.inl:298-300: 2 |halfsize|, CPU expression order
@@ -948,7 +932,6 @@ int nbl_geometry_destroy(nbl_geometry* g)
 	hipSetDevice(g->dev->ordinal);
 	hipFree(g->od);
 	hipFree(g->tris);
-	hipFree(g->tside);
 	delete g;
 	return NBL_OK;
 }

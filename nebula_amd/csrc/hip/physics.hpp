@@ -458,9 +458,6 @@ __device__ __forceinline__ Tri load_tri(const float4* t, int i)
 struct Geometry {
 	const int* od;
 	const float4* tris;
-	// per triangle: e1 x e2 (unnormalised, as the walk computes it) and the
-	// two materials as int16 halves (in: low, out: high)
-	const float4* tside;
 	F3 center, halfsize;
 	float max_extent;
 	int ntri;
@@ -569,15 +566,15 @@ __device__ inline int propagate(const Geometry& g, F3 start, F3 dir, float dista
 				break;
 			if (ti == ignore_tri)
 				continue;
-			// octree.inl:128-138: the side the ray leaves into; skip
-			// triangles whose outgoing material is the current one
-			const float4 sd = g.tside[ti];
-			const int mats = __float_as_int(sd.w);
-			const int mat_out = (sd.x * dir.x + sd.y * dir.y + sd.z * dir.z < 0.0f)
-				? (int)(short)(mats & 0xffff) : (mats >> 16);
+			// octree.inl:128-138: skip triangles whose outgoing material is
+			// the current one.  (A separate 16-byte record with e1 x e2
+			// and the materials, read before the triangle, measured 9%
+			// slower: the extra dependent load costs more than the cross
+			// product it saves.)
+			const Tri tri = load_tri(g.tris, ti);
+			const int mat_out = (dot(cross(tri.e1, tri.e2), dir) < 0.0f) ? tri.mat_in : tri.mat_out;
 			if (mat_out == ignore_mat)
 				continue;
-			const Tri tri = load_tri(g.tris, ti);
 			const float t = intersect_ray(tri, cur, dir);
 			if ((t > 0.0f) && (t <= isect + EPS)) {
 				isect = t;
