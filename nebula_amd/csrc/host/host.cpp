@@ -12,6 +12,7 @@
 #include <dlfcn.h>
 #include <fstream>
 #include <queue>
+#include <unordered_map>
 #include <random>
 #include <sstream>
 #include <string>
@@ -134,18 +135,6 @@ bool cell_overlap(const nbl_triangle44& t, V3 c, V3 h)
 	return tb.overlap(center, half, tv);
 }
 
-// core/triangle.inl:86-103
-bool perfect_overlap(const nbl_triangle44& a, const nbl_triangle44& b)
-{
-	const V3 a0 = tri_r0(a), a1 = tri_r1(a), a2 = tri_r2(a);
-	const V3 b0 = tri_r0(b), b1 = tri_r1(b), b2 = tri_r2(b);
-	return (veq(a0, b0) && veq(a1, b1) && veq(a2, b2))
-		|| (veq(a0, b1) && veq(a1, b2) && veq(a2, b0))
-		|| (veq(a0, b2) && veq(a1, b0) && veq(a2, b1))
-		|| (veq(a0, b0) && veq(a1, b2) && veq(a2, b1))
-		|| (veq(a0, b1) && veq(a1, b0) && veq(a2, b2))
-		|| (veq(a0, b2) && veq(a1, b1) && veq(a2, b0));
-}
 
 struct Node {
 	bool leaf = true;
@@ -172,23 +161,15 @@ struct Builder {
 	// overlapping it and its ancestors splits iff it holds more than 16 and
 	// d < 21 -- the fixed point of the one-at-a-time insertion of
 	// octree_builder.cpp:93-155 (a node's set only grows, leaf lists keep
-	// insertion order).
+	// insertion order).  Duplicates are removed once, before the build
+	// (see dedupe).
 	void build(int id, V3 c, V3 h, int depth, const std::vector<int32_t>& cand)
 	{
 		std::vector<int32_t> mine;
 		mine.reserve(cand.size());
-		for (int32_t ti : cand) {
-			if (!cell_overlap(tris[ti], c, h))
-				continue;
-			bool dup = false;
-			for (int32_t tj : mine)
-				if (perfect_overlap(tris[ti], tris[tj])) { dup = true; break; }
-			if (dup) {
-				std::fprintf(stderr, "WARNING: ignoring duplicate triangle!\n");
-				continue;
-			}
-			mine.push_back(ti);
-		}
+		for (int32_t ti : cand)
+			if (cell_overlap(tris[ti], c, h))
+				mine.push_back(ti);
 		if (mine.size() > split_count && depth < max_depth) {
 			nodes[id].leaf = false;
 			for (int o = 0; o < 8; ++o) {
@@ -205,6 +186,67 @@ struct Builder {
 		}
 	}
 };
+
+// octree_builder.cpp:100-111 drops a triangle inserted into a leaf that
+// already holds a perfect overlap of it (core/triangle.inl:86-103: the same
+// three vertices in any order).  Identical triangles overlap the same cells,
+// so this keeps, everywhere, the first triangle (by index) of each group of
+// perfect overlaps: one pass with a hash of the sorted vertex triple instead
+// of comparisons within every node.
+struct VKey {
+	float v[9];
+	bool operator==(const VKey& o) const
+	{
+		for (int k = 0; k < 9; ++k)
+			if (!(v[k] == o.v[k]))
+				return false;
+		return true;
+	}
+};
+struct VKeyHash {
+	size_t operator()(const VKey& k) const
+	{
+		uint64_t h = 1469598103934665603ull;
+		for (float f : k.v) {
+			uint32_t u;
+			std::memcpy(&u, &f, 4);
+			h = (h ^ u) * 1099511628211ull;
+		}
+		return (size_t)h;
+	}
+};
+
+std::vector<uint8_t> dedupe(const nbl_triangle44* tris, int32_t ntri)
+{
+	std::vector<uint8_t> keep((size_t)ntri, 1);
+	std::unordered_map<VKey, int32_t, VKeyHash> first;
+	first.reserve((size_t)ntri * 2);
+	for (int32_t i = 0; i < ntri; ++i) {
+		V3 p[3] = { tri_r0(tris[i]), tri_r1(tris[i]), tri_r2(tris[i]) };
+		bool nan = false;
+		for (V3& q : p) {
+			// -0 == +0 under ==; NaN equals nothing, so such a triangle is unique
+			if (q.x != q.x || q.y != q.y || q.z != q.z) nan = true;
+			if (q.x == 0.0f) q.x = 0.0f;
+			if (q.y == 0.0f) q.y = 0.0f;
+			if (q.z == 0.0f) q.z = 0.0f;
+		}
+		if (nan)
+			continue;
+		std::sort(p, p + 3, [](const V3& a, const V3& b) {
+			return a.x < b.x || (a.x == b.x && (a.y < b.y || (a.y == b.y && a.z < b.z)));
+		});
+		VKey k;
+		for (int j = 0; j < 3; ++j) {
+			k.v[3 * j] = p[j].x; k.v[3 * j + 1] = p[j].y; k.v[3 * j + 2] = p[j].z;
+		}
+		if (!first.emplace(k, i).second) {
+			keep[(size_t)i] = 0;
+			std::fprintf(stderr, "WARNING: ignoring duplicate triangle!\n");
+		}
+	}
+	return keep;
+}
 
 } // namespace
 
@@ -229,9 +271,10 @@ int build_octree(const nbl_triangle44* tris, int32_t ntri, std::vector<int32_t>&
 
 	Builder b;
 	b.tris = tris;
+	const std::vector<uint8_t> keep = dedupe(tris, ntri);
 	std::vector<int32_t> all;
 	for (int32_t i = 0; i < ntri; ++i)
-		if (cell_overlap(tris[i], c, h))
+		if (keep[(size_t)i] && cell_overlap(tris[i], c, h))
 			all.push_back(i);
 	b.nodes.emplace_back();
 	b.nodes[0].leaf = false;   // the root always has eight children

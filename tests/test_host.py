@@ -137,6 +137,49 @@ def test_product_builder_equals_oracle_builder(n, seed):
     assert len(a[1]) == n
 
 
+def test_product_builder_drops_duplicates_like_oracle():
+    """Perfectly overlapping triangles (same vertices in another order, or
+    with -0 for +0) are dropped as the reference's insertion builder drops
+    them (octree_builder.cpp:100-111): the first by index stays.  Enough
+    triangles to split many levels."""
+    base = _random_soup(400, 11)
+    v = base.copy()
+    dup = base[:60].copy()
+    r0 = dup["r0"].copy()
+    r1 = r0 + dup["e1"]
+    r2 = r0 + dup["e2"]
+    # rotated vertex order: (r1, r2, r0), and a mirrored one: (r0, r2, r1)
+    rot = dup.copy()
+    rot["r0"], rot["e1"], rot["e2"] = r1, r2 - r1, r0 - r1
+    mir = dup.copy()
+    mir["e1"], mir["e2"] = dup["e2"], dup["e1"]
+    zero = base[:1].copy()
+    zero["r0"] = (0.0, 0.0, 0.0)
+    negz = zero.copy()
+    negz["r0"] = (-0.0, 0.0, -0.0)
+    tris = np.concatenate([v, rot, mir, zero, negz, dup[:5]])
+    rng = np.random.default_rng(3)
+    tris = tris[rng.permutation(len(tris))]
+    a = A.build_octree(tris)
+    b = O.build_octree(tris)
+    assert np.array_equal(a[0], b[0])
+    assert a[1].tobytes() == b[1].tobytes()
+    assert len(a[1]) < len(tris)
+
+
+def test_product_builder_is_fast_on_200k_triangles():
+    """A 2e5-triangle contact-hole mesh builds in seconds (no pairwise
+    duplicate search within nodes)."""
+    import time
+    mesh = W.contact_hole_geometry(10, 10, n_seg=224, nz=2, layers=((80.0, 1),))
+    tris = A.mesh_to_triangles(mesh)
+    assert len(tris) > 150000
+    t = time.perf_counter()
+    od, st, c, h = A.build_octree(tris)
+    assert time.perf_counter() - t < 30.0
+    assert len(st) == len(tris)
+
+
 def test_octree_walk_equals_brute_force_on_soup():
     """geometry/octree.inl propagate vs the trilist scan of
     geometry/trilist.inl: same first hit except within the tolerance band."""
