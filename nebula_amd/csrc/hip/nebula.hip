@@ -1062,7 +1062,8 @@ int nbl_driver_create(nbl_device* dev, uint32_t capacity, const nbl_geometry* ge
 	}
 	if ((rc = alloc_bufs(d->buf[0], d->buf_mem[0], d->slots))) return cleanup(rc);
 	if ((rc = alloc_bufs(d->buf[1], d->buf_mem[1], d->slots))) return cleanup(rc);
-	d->spill_cap = std::max<uint32_t>(1u << 20, std::min<uint64_t>(4ull * d->slots, 1ull << 28));
+	// the spill stack starts at four buffers' worth and doubles when needed
+	d->spill_cap = std::max<uint32_t>(1024u, std::min<uint64_t>(4ull * d->slots, 1ull << 28));
 	if ((rc = alloc_bufs(d->spill, d->spill_mem, d->spill_cap))) return cleanup(rc);
 	if ((rc = alloc_bufs(d->inel, d->inel_mem, d->slots))) return cleanup(rc);
 	if ((rc = dev_alloc(&d->ctrl, 1))) return cleanup(rc);
@@ -1504,6 +1505,64 @@ int nbl_driver_set_launch(nbl_driver* d, uint32_t steps_per_launch, uint32_t blo
 	return NBL_OK;
 }
 
+namespace {
+// The spill stack holds secondaries that found no room in the particle
+// buffer.  Deep cascades (keV primaries make hundreds of secondaries each)
+// can fill it, and a particle whose secondary finds no room anywhere waits:
+// with the buffer and the stack full of such particles nothing would move.
+// Whenever the host reads the counters (stream idle), a stack more than
+// half full is doubled, so secondaries always find room.
+int grow_spill(nbl_driver* d)
+{
+	const Ctrl& c = *d->h_ctrl;
+	const uint32_t used = c.stepped ? std::min(c.spill_ctr, d->spill_cap) : c.spill_n;
+	if (used <= d->spill_cap / 2 || d->spill_cap >= (1u << 31))
+		return NBL_OK;
+	const uint32_t ncap = d->spill_cap * 2u;
+	Bufs nb;
+	void* nmem = nullptr;
+	int rc = alloc_bufs(nb, nmem, ncap);
+	if (rc) return rc;
+	double* nsc = nullptr;
+	if (d->map.np && (rc = dev_alloc(&nsc, (size_t)d->map.np * ncap))) {
+		hipFree(nmem);
+		return rc;
+	}
+	const size_t on = d->spill_cap, nn = ncap, u = used;
+	char* ob = d->spill.base;
+	char* nbb = nb.base;
+	if (u) {
+		HIP_OK(hipMemcpyAsync(nbb, ob, 8 * u, hipMemcpyDeviceToDevice, d->stream));
+		for (int f = 0; f < N_FIELDS; ++f)
+			HIP_OK(hipMemcpyAsync(nbb + 8 * nn + 4 * f * nn, ob + 8 * on + 4 * f * on, 4 * u,
+				hipMemcpyDeviceToDevice, d->stream));
+		for (int q = 0; q < d->map.np; ++q)
+			HIP_OK(hipMemcpyAsync(nsc + q * nn, d->spill.sc + q * on, 8 * u, hipMemcpyDeviceToDevice,
+				d->stream));
+	}
+	// secondaries beyond the old top were never stored (their parents wait
+	// and retry): the counter must not count them once the stack is larger
+	if (c.stepped && c.spill_ctr > used) {
+		const uint32_t top = used;
+		HIP_OK(hipMemcpyAsync(&d->ctrl->spill_ctr, &top, sizeof(top), hipMemcpyHostToDevice, d->stream));
+		d->h_ctrl->spill_ctr = top;
+	}
+	HIP_OK(hipStreamSynchronize(d->stream));
+	hipFree(d->spill_mem);
+	if (d->map.np)
+		hipFree(d->sc_mem[2]);
+	d->spill = nb;
+	d->spill_mem = nmem;
+	if (d->map.np) {
+		d->spill.sc = nsc;
+		d->sc_mem[2] = nsc;
+	}
+	d->spill_cap = ncap;
+	d->epoch++;
+	return NBL_OK;
+}
+} // namespace
+
 int nbl_driver_counts(nbl_driver* d, uint32_t* running, uint32_t* detected, uint64_t* queued)
 {
 	clear_error();
@@ -1512,6 +1571,8 @@ int nbl_driver_counts(nbl_driver* d, uint32_t* running, uint32_t* detected, uint
 	int rc = set_device(d->dev);
 	if (rc) return rc;
 	if ((rc = read_ctrl(d)))
+		return rc;
+	if ((rc = grow_spill(d)))
 		return rc;
 	const Ctrl& c = *d->h_ctrl;
 	if (running) *running = active_count(c, d->spill_cap);
