@@ -327,13 +327,15 @@ __device__ __forceinline__ bool place_secondary(const StepArgs& a, Ctrl* C, cons
 // code runs in k_inel on waves where every lane needs it (about 7% of the
 // steps of a 300 eV scan would otherwise put it into nearly every wave).
 //
-// Grid-stride over the input particles; each lane steps its particle up to
-// K times, stopping early when it leaves (detected, terminated, inelastic
-// event), and writes it out once: detected record, inelastic queue or
-// survivor buffer, each kind with one ballot and one atomic per wave.
-// (Lanes that took new work after every step or every few steps were
-// measured slower: the extra ballots and atomics cost more than the idle
-// lanes they saved; the limit is divergence within a step.)
+// Grid-stride over the input particles by whole waves; each lane steps its
+// particle up to K times, stopping early when it leaves (detected,
+// terminated, inelastic event), and writes it out once: detected record,
+// inelastic queue or survivor buffer, each kind with one ballot and one
+// atomic per wave.  The wave steps in lockstep because the octree walk is
+// shared across it (propagate_wave): a lane whose particle has stopped still
+// tests triangles for the others.  (Lanes that took new work after every step
+// or every few steps were measured slower: the extra ballots and atomics cost
+// more than the idle lanes they saved.)
 //
 // SC: also accumulate each particle's likelihood-ratio score (free-path
 // race and boundary choices) and write it with the particle.
@@ -373,65 +375,93 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	int* path = s_path + threadIdx.x;
 	uint32_t my_steps = 0;
 	const uint32_t stride = gridDim.x * blockDim.x;
-	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_in; i += stride) {
-		const uint32_t slot = (i < n_surv) ? i : (B - 1u - (i - n_surv));
-		State s = load_state(in, slot);
+	// Whole waves take consecutive particles and step in lockstep: the walk
+	// shares its triangle tests across the wave (propagate_wave), so every
+	// lane joins it each step, also lanes whose particle has stopped.
+	const uint32_t wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
+	for (uint32_t w0 = wave0; w0 < n_in; w0 += stride) {
+		const uint32_t i = w0 + __lane_id();
+		const bool valid = i < n_in;
+		State s = {};
 		Score sc;
-		if (SC) load_score(in, slot, np, sc);
-		if (MULTI) {
-			const WorldDesc& wd = a.worlds[s.flags >> FL_WORLD_SHIFT];
-			Wl.g = wd.g;
-			Wl.mats = wd.mats;
-			Wl.nmats = wd.nmats;
+		if (valid) {
+			const uint32_t slot = (i < n_surv) ? i : (B - 1u - (i - n_surv));
+			s = load_state(in, slot);
+			if (SC) load_score(in, slot, np, sc);
+			if (MULTI) {
+				const WorldDesc& wd = a.worlds[s.flags >> FL_WORLD_SHIFT];
+				Wl.g = wd.g;
+				Wl.mats = wd.mats;
+				Wl.nmats = wd.nmats;
+			}
 		}
-		bool det = false, to_inel = false, dead = false;
+		bool run = valid, det = false, to_inel = false, dead = false;
 		for (uint32_t k = 0; k < K; ++k) {
+			if (__ballot(run) == 0ull)
+				break;
 			int oc = ALIVE;
-			if (s.flags & FL_PEND_DET) {
-				oc = DETECTED;
-			} else if (s.flags & FL_PEND_INEL) {
-				// an inelastic event that found no secondary slot earlier
-				s.flags &= ~FL_PEND_INEL;
-				to_inel = true;
-			} else {
-				int ev = EV_NONE, hit = -1;
-				Rng rng;
-				rng_start(rng, W.key0, W.key1, s.sid, s.step);
-				const int mat0 = s.mat;
-				PathInfo pinfo;
-				oc = step_init(W, s, rng, ev, hit, path, 256, pinfo);
-				++my_steps;
-				if (SC && oc == ALIVE)
-					score_free_path(sc, a.map, W, mat0, pinfo, ev);
-				if (oc == ALIVE) {
-					if (ev == EV_INELASTIC) {
-						to_inel = true;
-					} else {
-						if (SC) {
-							float g = 0.0f;
-							int gmi = -1, gmo = -1;
-							const float e0 = s.p.E;
-							oc = step_event_elastic_or_boundary(W, s, rng, ev, hit, &g, &gmi, &gmo);
-							if (ev == EV_INTERSECT)
-								score_barrier(sc, a.map, gmi, gmo, g);
-							if (DEP_OK && a.dep)
-								deposit(a, C, ev == EV_ELASTIC && s.p.E != e0, s, e0);
-						} else {
-							const float e0 = s.p.E;
-							oc = step_event_elastic_or_boundary(W, s, rng, ev, hit);
-							if (DEP_OK && a.dep)
-								deposit(a, C, ev == EV_ELASTIC && s.p.E != e0, s, e0);
-						}
-					}
+			bool walk = false;
+			Rng rng;
+			int mat0 = s.mat, type = 0;
+			F3 nd = { 0.0f, 0.0f, 0.0f };
+			float distance = 0.0f;
+			PathInfo pinfo;
+			if (run) {
+				if (s.flags & FL_PEND_DET) {
+					oc = DETECTED;
+				} else if (s.flags & FL_PEND_INEL) {
+					// an inelastic event that found no secondary slot earlier
+					s.flags &= ~FL_PEND_INEL;
+					to_inel = true;
+				} else {
+					rng_start(rng, W.key0, W.key1, s.sid, s.step);
+					mat0 = s.mat;
+					oc = step_plan(W, s, rng, nd, distance, type, pinfo);
+					++my_steps;
+					walk = (oc == ALIVE);
 				}
 			}
-			if (oc == DETECTED) { det = true; break; }
-			if (oc == TERMINATED) { dead = true; break; }
-			if (to_inel)
-				break;
-			s.step++;
+			float hd;
+			const int hit = propagate_wave<!MULTI>(W.g, walk, s.p.pos, nd, distance, s.last_tri, s.mat, hd,
+				path, 256);
+			if (walk) {
+				int ev;
+				step_move(s, nd, distance, type, hit, hd, ev, pinfo);
+				if (SC)
+					score_free_path(sc, a.map, W, mat0, pinfo, ev);
+				if (ev == EV_INELASTIC) {
+					to_inel = true;
+				} else if (SC) {
+					float g = 0.0f;
+					int gmi = -1, gmo = -1;
+					const float e0 = s.p.E;
+					oc = step_event_elastic_or_boundary(W, s, rng, ev, hit, &g, &gmi, &gmo);
+					if (ev == EV_INTERSECT)
+						score_barrier(sc, a.map, gmi, gmo, g);
+					if (DEP_OK && a.dep)
+						deposit(a, C, ev == EV_ELASTIC && s.p.E != e0, s, e0);
+				} else {
+					const float e0 = s.p.E;
+					oc = step_event_elastic_or_boundary(W, s, rng, ev, hit);
+					if (DEP_OK && a.dep)
+						deposit(a, C, ev == EV_ELASTIC && s.p.E != e0, s, e0);
+				}
+			}
+			if (run) {
+				if (oc == DETECTED) {
+					det = true;
+					run = false;
+				} else if (oc == TERMINATED) {
+					dead = true;
+					run = false;
+				} else if (to_inel) {
+					run = false;
+				} else {
+					s.step++;
+				}
+			}
 		}
-		bool keep = !dead;
+		bool keep = valid && !dead;
 		const uint32_t dslot = wave_alloc(&C->det_ctr, det);
 		if (det) {
 			if (dslot < det_cap) {
@@ -470,12 +500,12 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	count_steps(C, my_steps);
 }
 
-// The plain variant runs at 7 waves per SIMD (the LDS path cache allows 7
-// blocks per compute unit; 72 VGPRs and a 20-byte spill measured faster
-// than 6 waves at 80 VGPRs)
-// (the scored and multi-world variants would spill, so they are left free).
+// The plain variant runs at 5 waves per SIMD (96 VGPRs, a 20-byte spill):
+// measured faster than 6 waves (80 VGPRs, 84-byte spill) or 7 (72 VGPRs,
+// 128-byte spill).  The scored and multi-world variants would spill, so
+// they are left free.
 #ifndef NBL_PLAIN_WAVES
-#define NBL_PLAIN_WAVES 7
+#define NBL_PLAIN_WAVES 5
 #endif
 template <bool SC, bool MULTI>
 __global__ __launch_bounds__(256) void k_step(StepArgs a)
@@ -557,13 +587,24 @@ __global__ __launch_bounds__(256) void k_query(Geometry g, const float* pos, con
 {
 	__shared__ int s_path[PATH_LEVELS * 256];
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
-	const F3 p = { pos[3 * i], pos[3 * i + 1], pos[3 * i + 2] };
-	const F3 d = { dir[3 * i], dir[3 * i + 1], dir[3 * i + 2] };
+	const bool valid = i < n;
+	F3 p = { 0.0f, 0.0f, 0.0f }, d = { 1.0f, 0.0f, 0.0f };
+	float dd = 0.0f;
+	int it = -1, im = 0;
+	if (valid) {
+		p = { pos[3 * i], pos[3 * i + 1], pos[3 * i + 2] };
+		d = { dir[3 * i], dir[3 * i + 1], dir[3 * i + 2] };
+		dd = dist[i];
+		it = ignore_tri[i];
+		im = ignore_mat[i];
+	}
+	// the walk of the step kernel: whole waves, candidate tests shared
 	float od;
-	out_tri[i] = propagate(g, p, d, dist[i], ignore_tri[i], ignore_mat[i], od, s_path + threadIdx.x, 256);
-	out_dist[i] = od;
+	const int r = propagate_wave<true>(g, valid, p, d, dd, it, im, od, s_path + threadIdx.x, 256);
+	if (valid) {
+		out_tri[i] = r;
+		out_dist[i] = od;
+	}
 }
 
 // Test hook: table lookups.
@@ -646,6 +687,7 @@ struct nbl_geometry {
 	nbl_device* dev;
 	int* od;
 	float4* tris;
+	int* cnt;
 	Geometry g;
 	int max_material;   // largest physical material index named by a triangle
 };
@@ -892,11 +934,18 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 	if (rc) return rc;
 	auto* g = new nbl_geometry();
 	g->dev = dev;
-	if ((rc = dev_alloc(&g->od, (size_t)n_od)) || (rc = dev_alloc(&g->tris, (size_t)ntri * 3))) {
+	if ((rc = dev_alloc(&g->od, (size_t)n_od)) || (rc = dev_alloc(&g->tris, (size_t)ntri * 3))
+		|| (rc = dev_alloc(&g->cnt, (size_t)n_od))) {
 		hipFree(g->od);
+		hipFree(g->tris);
 		delete g;
 		return rc;
 	}
+	// cnt[i]: number of triangle entries from od[i] up to the next negative
+	// entry (the length of a leaf list when i is its start)
+	std::vector<int32_t> cnt((size_t)n_od, 0);
+	for (int64_t i = n_od - 1; i >= 0; --i)
+		cnt[i] = (od[i] < 0 || i + 1 == n_od) ? 0 : cnt[i + 1] + 1;
 	std::vector<float4> packed((size_t)ntri * 3);
 	for (int64_t i = 0; i < ntri; ++i) {
 		const nbl_triangle44& t = tris[i];
@@ -909,8 +958,10 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 	}
 	HIP_OK(hipMemcpy(g->od, od, sizeof(int32_t) * n_od, hipMemcpyHostToDevice));
 	HIP_OK(hipMemcpy(g->tris, packed.data(), sizeof(float4) * packed.size(), hipMemcpyHostToDevice));
+	HIP_OK(hipMemcpy(g->cnt, cnt.data(), sizeof(int32_t) * n_od, hipMemcpyHostToDevice));
 	g->g.od = g->od;
 	g->g.tris = g->tris;
+	g->g.cnt = g->cnt;
 	g->g.center = { center[0], center[1], center[2] };
 	g->g.halfsize = { halfsize[0], halfsize[1], halfsize[2] };
 	// octree.inl:298-300: 2 |halfsize|, CPU expression order
@@ -932,6 +983,7 @@ int nbl_geometry_destroy(nbl_geometry* g)
 	hipSetDevice(g->dev->ordinal);
 	hipFree(g->od);
 	hipFree(g->tris);
+	hipFree(g->cnt);
 	delete g;
 	return NBL_OK;
 }

@@ -440,7 +440,9 @@ struct Material {
 
 // -------------------------------------------------------------- geometry --
 // Triangle as three float4: (r0.xyz, e1.x), (e1.yz, e2.xy), (e2.z, mat_in,
-// mat_out, 0) with the materials stored as int bits.
+// mat_out, 0) with the materials stored as int bits.  (A 64-byte record that
+// also holds e1 x e2 measured slower: the extra load costs more than the
+// cross product it saves.)
 struct Tri { F3 r0, e1, e2; int mat_in, mat_out; };
 
 __device__ __forceinline__ Tri load_tri(const float4* t, int i)
@@ -458,6 +460,7 @@ __device__ __forceinline__ Tri load_tri(const float4* t, int i)
 struct Geometry {
 	const int* od;
 	const float4* tris;
+	const int* cnt;  // cnt[i]: triangles listed from od[i] to the list's end
 	F3 center, halfsize;
 	float max_extent;
 	int ntri;
@@ -497,91 +500,133 @@ __device__ __forceinline__ bool in_domain(const Geometry& g, F3 p)
 // node index at level l (1..21) of the current cell.
 constexpr int PATH_LEVELS = 22;
 
-// geometry/octree.inl:32-192: walk leaf cells by location code; returns the
-// hit triangle (or -1) and the distance.
+// The pieces of the cell walk of geometry/octree.inl:32-192.
 //
 // The reference re-descends from the root for every cell.  Here the node
 // indices of the current cell's ancestors are kept in LDS (`path`, one int
 // per level, lane-strided), so moving to a neighbour loads only the flipped
 // level and whatever lies below it.  The cell-centre arithmetic still runs
 // from the root in the reference's order, so every value is unchanged.
+//
+// cell_find: centre (ax, ay, az) and scale aw of the leaf holding `cur`,
+// extending `location` down to it; returns the start of its triangle list.
+__device__ __forceinline__ int cell_find(const Geometry& g, F3 cur, uint64_t& location,
+	float& ax, float& ay, float& az, float& aw, int* __restrict__ path, int stride)
+{
+	const int* __restrict__ od = g.od;
+	const F3 hs = g.halfsize;
+	ax = g.center.x; ay = g.center.y; az = g.center.z; aw = 1.0f;
+	const int clz = __clzll((long long)location);
+	for (int i = 60 - clz; i >= 0; i -= 3) {
+		const unsigned oct = (unsigned)(location >> i) & 7u;
+		ax += aw * hs.x * ((float)(oct & 1u) - 0.5f);
+		ay += aw * hs.y * ((float)((oct & 2u) >> 1) - 0.5f);
+		az += aw * hs.z * ((float)((oct & 4u) >> 2) - 0.5f);
+		aw *= 0.5f;
+	}
+	// levels 1..L-1 are unchanged ancestors; level L holds the new octant
+	int lev = (63 - clz) / 3;
+	int index = 0;
+	if (lev > 0) {
+		const int parent = (lev >= 2) ? path[(lev - 1) * stride] : 0;
+		index = od[parent + (int)((unsigned)location & 7u)];
+		path[lev * stride] = index;
+	}
+	while (index >= 0) {
+		unsigned oct = 0;
+		oct ^= (cur.x > ax) ? 1u : 0u;
+		oct ^= (cur.y > ay) ? 2u : 0u;
+		oct ^= (cur.z > az) ? 4u : 0u;
+		location = (location << 3) | oct;
+		index = od[index + (int)oct];
+		path[++lev * stride] = index;
+		ax += aw * hs.x * ((float)(oct & 1u) - 0.5f);
+		ay += aw * hs.y * ((float)((oct & 2u) >> 1) - 0.5f);
+		az += aw * hs.z * ((float)((oct & 4u) >> 2) - 0.5f);
+		aw *= 0.5f;
+	}
+	return -index;
+}
+
+// Distance to the cell wall the ray leaves through (target -1/-2/-4 for the
+// x/y/z wall).
+__device__ __forceinline__ void cell_exit(const Geometry& g, F3 cur, F3 dir, float ax, float ay, float az,
+	float aw, float& isect, int& target)
+{
+	const F3 hs = g.halfsize;
+	const float hx = aw * hs.x, hy = aw * hs.y, hz = aw * hs.z;
+	const float tx = (ax + copysignf(hx + EPS, dir.x) - cur.x) / dir.x;
+	const float ty = (ay + copysignf(hy + EPS, dir.y) - cur.y) / dir.y;
+	const float tz = (az + copysignf(hz + EPS, dir.z) - cur.z) / dir.z;
+	if ((tx < ty) && (tx < tz)) { isect = tx; target = -1; }
+	else if ((ty < tx) && (ty < tz)) { isect = ty; target = -2; }
+	else { isect = tz; target = -4; }
+}
+
+// Candidate test: octree.inl:128-138 skips triangles whose outgoing material
+// is the current one, then core/triangle.inl gives the distance (or -1).
+// (A separate 16-byte record with e1 x e2 and the materials, read before the
+// triangle, measured 9% slower: the extra dependent load costs more than the
+// cross product it saves.  Estimating the three quotients with one
+// reciprocal and dividing only where the outcome is not certain kept every
+// result but measured 3% slower in the wave-wide walk.)
+__device__ __forceinline__ float cell_candidate(const Geometry& g, int ti, F3 cur, F3 dir, int ignore_mat)
+{
+	const Tri tri = load_tri(g.tris, ti);
+	const int mat_out = (dot(cross(tri.e1, tri.e2), dir) < 0.0f) ? tri.mat_in : tri.mat_out;
+	if (mat_out == ignore_mat)
+		return -1.0f;
+	return intersect_ray(tri, cur, dir);
+}
+
+// The neighbour across wall `target` (-1/-2/-4): flip that axis' bit at the
+// deepest level where the move stays inside the parent; location 1 (or 0)
+// means the ray left the octree.
+__device__ __forceinline__ void cell_next(uint64_t& location, int target, F3 dir)
+{
+	const unsigned mask = (unsigned)(-target);
+	unsigned value;
+	if (mask == 1u) value = (dir.x >= 0.0f) ? 0u : 1u;
+	else if (mask == 2u) value = (dir.y >= 0.0f) ? 0u : 2u;
+	else value = (dir.z >= 0.0f) ? 0u : 4u;
+	while (location > 1) {
+		if ((location & (uint64_t)mask) == (uint64_t)value) {
+			location ^= (uint64_t)mask;
+			break;
+		}
+		location >>= 3;
+	}
+}
+
+// geometry/octree.inl:32-192: walk leaf cells by location code; returns the
+// hit triangle (or -1) and the distance.  One lane, one ray.
 __device__ inline int propagate(const Geometry& g, F3 start, F3 dir, float distance,
 	int ignore_tri, int ignore_mat, float& out_dist, int* __restrict__ path, int stride)
 {
 	const int* __restrict__ od = g.od;
-	const F3 hs = g.halfsize;
 	F3 cur = start;
 	float to_go = distance;
 	uint64_t location = 1;
 	uint32_t cells = 0;
 	do {
-		float ax = g.center.x, ay = g.center.y, az = g.center.z, aw = 1.0f;
-		const int clz = __clzll((long long)location);
-		for (int i = 60 - clz; i >= 0; i -= 3) {
-			const unsigned oct = (unsigned)(location >> i) & 7u;
-			ax += aw * hs.x * ((float)(oct & 1u) - 0.5f);
-			ay += aw * hs.y * ((float)((oct & 2u) >> 1) - 0.5f);
-			az += aw * hs.z * ((float)((oct & 4u) >> 2) - 0.5f);
-			aw *= 0.5f;
-		}
-		// levels 1..L-1 are unchanged ancestors; level L holds the new octant
-		int lev = (63 - clz) / 3;
-		int index = 0;
-		if (lev > 0) {
-			const int parent = (lev >= 2) ? path[(lev - 1) * stride] : 0;
-			index = od[parent + (int)((unsigned)location & 7u)];
-			path[lev * stride] = index;
-		}
-		while (index >= 0) {
-			unsigned oct = 0;
-			oct ^= (cur.x > ax) ? 1u : 0u;
-			oct ^= (cur.y > ay) ? 2u : 0u;
-			oct ^= (cur.z > az) ? 4u : 0u;
-			location = (location << 3) | oct;
-			index = od[index + (int)oct];
-			path[++lev * stride] = index;
-			ax += aw * hs.x * ((float)(oct & 1u) - 0.5f);
-			ay += aw * hs.y * ((float)((oct & 2u) >> 1) - 0.5f);
-			az += aw * hs.z * ((float)((oct & 4u) >> 2) - 0.5f);
-			aw *= 0.5f;
-		}
-		index = -index;
+		float ax, ay, az, aw;
+		int index = cell_find(g, cur, location, ax, ay, az, aw, path, stride);
 		++cells;
-
 		float isect;
 		int target;
-		{
-			const float hx = aw * hs.x, hy = aw * hs.y, hz = aw * hs.z;
-			const float tx = (ax + copysignf(hx + EPS, dir.x) - cur.x) / dir.x;
-			const float ty = (ay + copysignf(hy + EPS, dir.y) - cur.y) / dir.y;
-			const float tz = (az + copysignf(hz + EPS, dir.z) - cur.z) / dir.z;
-			if ((tx < ty) && (tx < tz)) { isect = tx; target = -1; }
-			else if ((ty < tx) && (ty < tz)) { isect = ty; target = -2; }
-			else { isect = tz; target = -4; }
-		}
-
+		cell_exit(g, cur, dir, ax, ay, az, aw, isect, target);
 		for (;;) {
 			const int ti = od[index++];
 			if (ti < 0)
 				break;
 			if (ti == ignore_tri)
 				continue;
-			// octree.inl:128-138: skip triangles whose outgoing material is
-			// the current one.  (A separate 16-byte record with e1 x e2
-			// and the materials, read before the triangle, measured 9%
-			// slower: the extra dependent load costs more than the cross
-			// product it saves.)
-			const Tri tri = load_tri(g.tris, ti);
-			const int mat_out = (dot(cross(tri.e1, tri.e2), dir) < 0.0f) ? tri.mat_in : tri.mat_out;
-			if (mat_out == ignore_mat)
-				continue;
-			const float t = intersect_ray(tri, cur, dir);
+			const float t = cell_candidate(g, ti, cur, dir, ignore_mat);
 			if ((t > 0.0f) && (t <= isect + EPS)) {
 				isect = t;
 				target = ti;
 			}
 		}
-
 		if (isect >= to_go) {
 			out_dist = distance;
 			return -1;
@@ -591,22 +636,155 @@ __device__ inline int propagate(const Geometry& g, F3 start, F3 dir, float dista
 		}
 		to_go -= isect;
 		cur = v_add(cur, v_mul(dir, isect));
-
-		const unsigned mask = (unsigned)(-target);
-		unsigned value;
-		if (mask == 1u) value = (dir.x >= 0.0f) ? 0u : 1u;
-		else if (mask == 2u) value = (dir.y >= 0.0f) ? 0u : 2u;
-		else value = (dir.z >= 0.0f) ? 0u : 4u;
-		while (location > 1) {
-			if ((location & (uint64_t)mask) == (uint64_t)value) {
-				location ^= (uint64_t)mask;
-				break;
-			}
-			location >>= 3;
-		}
+		cell_next(location, target, dir);
 	} while (location > 1 && cells < MAX_CELLS);
 	out_dist = distance;
 	return -1;
+}
+
+// Lane `src`'s value of a pointer.
+__device__ __forceinline__ const void* shfl_ptr(const void* p, int src)
+{
+	const uint64_t v = (uint64_t)p;
+	const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src);
+	const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src);
+	return (const void*)(((uint64_t)hi << 32) | lo);
+}
+
+// Sum of v over lanes 0..lane; every lane of the wave must be active.
+__device__ __forceinline__ int wave_incl_sum(int v)
+{
+	const int lane = (int)__lane_id();
+#pragma unroll
+	for (int d = 1; d < 64; d <<= 1) {
+		const int t = __shfl_up(v, d);
+		if (lane >= d)
+			v += t;
+	}
+	return v;
+}
+
+// The same walk for the rays of a whole wave, with the candidate tests shared
+// out over all 64 lanes.
+//
+// One lane walking its own cell's triangle list leaves most of the wave idle:
+// lists differ in length, and the wave runs as long as the longest.  Here
+// every round, each walking lane enters its next cell and counts its list
+// (g.cnt); the (ray, candidate) pairs of all lists are numbered by a prefix
+// sum over lanes and tested 64 at a time, one per lane, with the owning
+// ray's origin and direction fetched across lanes.  Each owner then takes,
+// in list order, the results that can hit (t > 0) and applies the
+// reference's running-minimum rule to them, so hit, distance and tie
+// choices are those of `propagate`.
+//
+// Every lane of the wave must call it (lanes with walk == false take part in
+// the shared tests only); a call from a partial wave falls back to
+// `propagate` lane by lane.
+//
+// SHARED: every lane walks the same geometry.  Otherwise (lanes of different
+// scans) the candidate tests read the owning lane's triangle tables.
+template <bool SHARED>
+__device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 dir, float distance,
+	int ignore_tri, int ignore_mat, float& out_dist, int* __restrict__ path, int stride)
+{
+	out_dist = distance;
+	if (__ballot(1) != ~0ull) {
+		int r = -1;
+		if (walk)
+			r = propagate(g, start, dir, distance, ignore_tri, ignore_mat, out_dist, path, stride);
+		return r;
+	}
+	const int lane = (int)__lane_id();
+	F3 cur = start;
+	float to_go = distance;
+	uint64_t location = 1;
+	uint32_t cells = 0;
+	int result = -1;
+	bool active = walk;
+	while (__ballot(active) != 0ull) {
+		int n = 0, off = 0, target = 0;
+		float isect = 0.0f;
+		if (active) {
+			float ax, ay, az, aw;
+			off = cell_find(g, cur, location, ax, ay, az, aw, path, stride);
+			++cells;
+			cell_exit(g, cur, dir, ax, ay, az, aw, isect, target);
+			n = g.cnt[off];
+		}
+		const int incl = wave_incl_sum(n);
+		const int excl = incl - n;
+		off -= excl;  // pair p of this lane tests od[off + p]
+		const int total = __builtin_amdgcn_readlane(incl, 63);
+		for (int base = 0; base < total; base += 64) {
+			const int p = base + lane;
+			// owner of pair p: the last lane whose pairs start at or before p
+			int o = 0;
+#pragma unroll
+			for (int st = 32; st > 0; st >>= 1) {
+				if (__shfl(excl, o + st) <= p)
+					o += st;
+			}
+			const F3 oc = { __shfl(cur.x, o), __shfl(cur.y, o), __shfl(cur.z, o) };
+			const F3 odir = { __shfl(dir.x, o), __shfl(dir.y, o), __shfl(dir.z, o) };
+			const int oign_tri = __shfl(ignore_tri, o), oign_mat = __shfl(ignore_mat, o);
+			const int ooff = __shfl(off, o);
+			Geometry og = g;
+			if (!SHARED) {
+				og.od = (const int*)shfl_ptr(g.od, o);
+				og.tris = (const float4*)shfl_ptr(g.tris, o);
+			}
+			float t = -1.0f;
+			int ti = -1;
+			if (p < total) {
+				ti = og.od[ooff + p];
+				if (ti != oign_tri)
+					t = cell_candidate(og, ti, oc, odir, oign_mat);
+			}
+			// this lane's pairs within the chunk, lanes [lo, hi)
+			uint64_t todo = 0;
+			const uint64_t can_hit = __ballot(t > 0.0f);
+			if (active) {
+				const int lo = max(excl - base, 0), hi = min(incl - base, 64);
+				if (lo < hi) {
+					const uint64_t upto = (hi == 64) ? ~0ull : ((1ull << hi) - 1ull);
+					todo = can_hit & upto & ~((1ull << lo) - 1ull);
+				}
+			}
+			while (__ballot(todo != 0ull) != 0ull) {
+				const int src = todo ? (int)__builtin_ctzll(todo) : lane;
+				const float ts = __shfl(t, src);
+				const int tis = __shfl(ti, src);
+				if (todo) {
+					if (ts <= isect + EPS) {
+						isect = ts;
+						target = tis;
+					}
+					todo &= todo - 1ull;
+				}
+			}
+		}
+		if (active) {
+			if (isect >= to_go) {
+				out_dist = distance;
+				result = -1;
+				active = false;
+			} else if (target >= 0) {
+				out_dist = distance - to_go + isect;
+				result = target;
+				active = false;
+			} else {
+				to_go -= isect;
+				cur = v_add(cur, v_mul(dir, isect));
+				cell_next(location, target, dir);
+				if (!(location > 1 && cells < MAX_CELLS)) {
+					out_dist = distance;
+					result = -1;
+					active = false;
+				}
+			}
+		}
+	}
+	return result;
 }
 
 // -------------------------------------------------------------- physics --
@@ -935,19 +1113,19 @@ struct WorldDesc {
 	int nmats;
 };
 
-// init (drivers/gpu/gpu_driver.inl:403-480): domain and energy checks, path
-// sampling, propagate, move; stores the normalised direction.  Returns ALIVE
-// or TERMINATED and the event; `hit` is the intersected triangle.
-__device__ inline int step_init(const World& W, State& s, Rng& rng, int& event, int& hit,
-	int* path, int stride, PathInfo& pinfo)
+// init (drivers/gpu/gpu_driver.inl:403-480) in two halves around the walk.
+//
+// step_plan: domain and energy checks and path sampling.  Returns ALIVE or
+// TERMINATED; for ALIVE, `distance` and `type` are the free path and its
+// mechanism (0 in vacuum), `nd` the normalised direction.
+__device__ __forceinline__ int step_plan(const World& W, const State& s, Rng& rng, F3& nd, float& distance,
+	int& type, PathInfo& pinfo)
 {
-	event = EV_NONE;
-	hit = -1;
 	pinfo.type = 0;
+	type = 0;
 	if (!in_domain(W.g, s.p.pos))
 		return TERMINATED;
-	int type = 0;
-	float distance = W.g.max_extent;
+	distance = W.g.max_extent;
 	float tau_el = 0.0f, tau_in = 0.0f;
 	if (s.mat >= 0) {
 		const Material& m = W.mats[s.mat];
@@ -962,9 +1140,17 @@ __device__ inline int step_init(const World& W, State& s, Rng& rng, int& event, 
 		if (pi < dist) { type = 1; dist = pi; }
 		distance = dist;
 	}
-	const F3 nd = normalised(s.p.dir);
-	float hd;
-	hit = propagate(W.g, s.p.pos, nd, distance, s.last_tri, s.mat, hd, path, stride);
+	nd = normalised(s.p.dir);
+	pinfo.tau_el = tau_el;
+	pinfo.tau_in = tau_in;
+	return ALIVE;
+}
+
+// step_move: after the walk, move to the hit or to the end of the free path
+// and name the event; stores the normalised direction.
+__device__ __forceinline__ void step_move(State& s, F3 nd, float distance, int type, int hit, float hd,
+	int& event, PathInfo& pinfo)
+{
 	s.p.dir = nd;
 	if (hit < 0) {
 		s.p.pos = v_add(s.p.pos, v_mul(nd, distance));
@@ -974,10 +1160,26 @@ __device__ inline int step_init(const World& W, State& s, Rng& rng, int& event, 
 		s.last_tri = hit;
 		event = EV_INTERSECT;
 	}
-	pinfo.tau_el = tau_el;
-	pinfo.tau_in = tau_in;
 	pinfo.dist = (hit < 0) ? distance : hd;
 	pinfo.type = (s.mat >= 0) ? type : 0;
+}
+
+// The whole init for one lane: domain and energy checks, path sampling,
+// propagate, move.  Returns ALIVE or TERMINATED and the event; `hit` is the
+// intersected triangle.
+__device__ inline int step_init(const World& W, State& s, Rng& rng, int& event, int& hit,
+	int* path, int stride, PathInfo& pinfo)
+{
+	event = EV_NONE;
+	hit = -1;
+	F3 nd;
+	float distance;
+	int type;
+	if (step_plan(W, s, rng, nd, distance, type, pinfo) != ALIVE)
+		return TERMINATED;
+	float hd;
+	hit = propagate(W.g, s.p.pos, nd, distance, s.last_tri, s.mat, hd, path, stride);
+	step_move(s, nd, distance, type, hit, hd, event, pinfo);
 	return ALIVE;
 }
 
