@@ -66,21 +66,33 @@ struct ScoreMap {
 struct Ctrl {
 	uint32_t buf_in;        // buffer the next k_step reads
 	uint32_t n_surv, n_se;  // particles in buf_in
-	uint32_t surv_ctr, se_ctr, se_limit;  // output counters of the running k_step
+	uint32_t se_limit;      // secondaries that fit the buffer's SE region
 	uint32_t stepped;       // a k_step ran since the last k_ctrl
 	uint32_t refill_n, refill_dst;
 	uint32_t target;        // particles in flight to aim for
-	uint32_t det_ctr;       // detected records written (may overshoot det_cap)
 	uint32_t spill_n;       // secondaries parked on the spill stack
-	uint32_t spill_ctr;     // spill stack top during a k_step (may overshoot)
 	uint32_t spill_take, spill_src;  // refill from the spill stack
-	uint32_t inel_ctr;      // particles queued for their inelastic event
 	uint32_t work_ctr;      // unused
-	uint32_t dep_ctr;       // energy-deposit records written (may overshoot dep_cap)
 	uint64_t refill_src;
 	uint64_t queue_n, queue_next;
-	uint64_t steps;         // particle-steps (init calls on active particles)
+	// Counters that waves of the step kernels bump atomically.  Each sits on
+	// a 128-byte line of its own (atomics to one line are serialised in L2),
+	// and the two a wave always needs together share one 64-bit word, so
+	// that one atomic returns both bases.
+	alignas(128) uint64_t step_out;   // k_step: survivors (low 32) | inelastic queue (high 32)
+	alignas(128) uint64_t inel_out;   // k_inel: secondaries (low 32) | parents kept (high 32)
+	alignas(128) uint32_t det_ctr;    // detected records written (may overshoot det_cap)
+	alignas(128) uint32_t spill_ctr;  // spill stack top during a k_step (may overshoot)
+	alignas(128) uint32_t dep_ctr;    // energy-deposit records written (may overshoot dep_cap)
+	alignas(128) uint64_t steps;      // particle-steps (init calls on active particles)
 };
+
+// Counts held in the packed words.
+__host__ __device__ inline uint32_t lo32(uint64_t v) { return (uint32_t)v; }
+__host__ __device__ inline uint32_t hi32(uint64_t v) { return (uint32_t)(v >> 32); }
+// survivors written by the last k_step + k_inel pair, secondaries requested
+__host__ __device__ inline uint32_t out_surv(const Ctrl& c) { return lo32(c.step_out) + hi32(c.inel_out); }
+__host__ __device__ inline uint32_t out_se(const Ctrl& c) { return lo32(c.inel_out); }
 
 struct StepArgs {
 	World W;
@@ -200,6 +212,28 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t* ctr, bool want)
 	return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
 
+// Two slot kinds at once from a packed counter (kind a in the low 32 bits,
+// kind b in the high 32 bits): one atomic per wave for both.
+__device__ __forceinline__ void wave_alloc2(uint64_t* ctr, bool want_a, bool want_b, uint32_t& slot_a,
+	uint32_t& slot_b)
+{
+	const uint64_t ma = __ballot(want_a), mb = __ballot(want_b);
+	slot_a = slot_b = 0;
+	if ((ma | mb) == 0)
+		return;
+	const uint32_t lane = __lane_id();
+	const int leader = __ffsll((unsigned long long)(ma | mb)) - 1;
+	uint64_t base = 0;
+	if ((int)lane == leader)
+		base = atomicAdd((unsigned long long*)ctr,
+			((unsigned long long)__popcll(mb) << 32) | (unsigned long long)__popcll(ma));
+	const uint32_t blo = (uint32_t)__shfl((int)lo32(base), leader);
+	const uint32_t bhi = (uint32_t)__shfl((int)hi32(base), leader);
+	const uint64_t below = (1ull << lane) - 1ull;
+	slot_a = blo + (uint32_t)__popcll(ma & below);
+	slot_b = bhi + (uint32_t)__popcll(mb & below);
+}
+
 // ------------------------------------------------------------- kernels --
 // Adopt the previous launch's output and plan the refill from the queue.
 // Secondaries that found no room in the particle buffer wait on the spill
@@ -210,8 +244,8 @@ __global__ void k_ctrl(Ctrl* c, uint32_t slots, uint32_t spill_cap)
 		return;
 	if (c->stepped) {
 		c->buf_in ^= 1u;
-		c->n_surv = c->surv_ctr;
-		c->n_se = min(c->se_ctr, c->se_limit);
+		c->n_surv = out_surv(*c);
+		c->n_se = min(out_se(*c), c->se_limit);
 		c->spill_n = min(c->spill_ctr, spill_cap);
 		c->stepped = 0;
 	}
@@ -231,9 +265,8 @@ __global__ void k_ctrl(Ctrl* c, uint32_t slots, uint32_t spill_cap)
 	c->queue_next += m;
 	c->n_surv += take + m;
 	c->se_limit = slots - (c->n_surv + c->n_se);
-	c->surv_ctr = 0;
-	c->se_ctr = 0;
-	c->inel_ctr = 0;
+	c->step_out = 0;
+	c->inel_out = 0;
 	c->work_ctr = 0;
 	c->spill_ctr = c->spill_n;
 	c->stepped = 1;
@@ -293,9 +326,8 @@ __device__ __forceinline__ void count_steps(Ctrl* C, uint32_t my_steps)
 // starts from its parent's score.
 template <bool SC>
 __device__ __forceinline__ bool place_secondary(const StepArgs& a, Ctrl* C, const Bufs& out,
-	uint32_t se_limit, bool made, const Particle& se, const State& parent, const Score& psc)
+	uint32_t se_limit, bool made, uint32_t sslot, const Particle& se, const State& parent, const Score& psc)
 {
-	const uint32_t sslot = wave_alloc(&C->se_ctr, made);
 	bool ok = true;
 	if (made) {
 		State ns;
@@ -485,13 +517,13 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 			}
 		}
 		const bool queue = keep && to_inel;
-		const uint32_t qslot = wave_alloc(&C->inel_ctr, queue);
+		const bool survive = keep && !to_inel;
+		uint32_t oslot, qslot;
+		wave_alloc2(&C->step_out, survive, queue, oslot, qslot);
 		if (queue) {
 			store_state(a.inel, qslot, s);
 			if (SC) store_score(a.inel, qslot, np, sc);
 		}
-		const bool survive = keep && !to_inel;
-		const uint32_t oslot = wave_alloc(&C->surv_ctr, survive);
 		if (survive) {
 			store_state(out, oslot, s);
 			if (SC) store_score(out, oslot, np, sc);
@@ -529,7 +561,8 @@ __global__ __launch_bounds__(256) void k_inel(StepArgs a)
 	Ctrl* C = a.ctrl;
 	const uint32_t bi = C->buf_in;
 	const Bufs out = bi ? a.buf[0] : a.buf[1];
-	const uint32_t n = C->inel_ctr, se_limit = C->se_limit;
+	// parents go on after k_step's survivors
+	const uint32_t n = hi32(C->step_out), se_limit = C->se_limit, surv0 = lo32(C->step_out);
 	const int np = a.map.np;
 	World Wl = a.W;
 	const World& W = MULTI ? Wl : a.W;
@@ -551,7 +584,10 @@ __global__ __launch_bounds__(256) void k_inel(StepArgs a)
 		bool made = false;
 		step_event(W, s, rng, EV_INELASTIC, -1, se, made);
 		bool done = true;
-		if (place_secondary<SC>(a, C, out, se_limit, made, se, s, sc)) {
+		uint32_t sslot, oslot;
+		wave_alloc2(&C->inel_out, made, true, sslot, oslot);
+		oslot += surv0;
+		if (place_secondary<SC>(a, C, out, se_limit, made, sslot, se, s, sc)) {
 			s.step++;
 		} else {
 			// no room anywhere: retry the event from the same draws later
@@ -561,7 +597,6 @@ __global__ __launch_bounds__(256) void k_inel(StepArgs a)
 		}
 		if ((SC || MULTI) && a.dep)
 			deposit(a, C, done && s.p.E != pre.p.E, s, pre.p.E);
-		const uint32_t oslot = wave_alloc(&C->surv_ctr, true);
 		store_state(out, oslot, s);
 		if (SC) store_score(out, oslot, np, sc);
 	}
@@ -828,7 +863,7 @@ int read_ctrl(nbl_driver* d)
 uint32_t active_count(const Ctrl& c, uint32_t spill_cap)
 {
 	if (c.stepped)
-		return c.surv_ctr + std::min(c.se_ctr, c.se_limit) + std::min(c.spill_ctr, spill_cap);
+		return out_surv(c) + std::min(out_se(c), c.se_limit) + std::min(c.spill_ctr, spill_cap);
 	return c.n_surv + c.n_se + c.spill_n;
 }
 
