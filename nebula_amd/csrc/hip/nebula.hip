@@ -404,7 +404,9 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	World Wl = a.W;
 	const World& W = MULTI ? Wl : a.W;
 	__shared__ int s_path[PATH_LEVELS * 256];
+	__shared__ int s_owner[256];
 	int* path = s_path + threadIdx.x;
+	int* owner_at = s_owner + (threadIdx.x & ~63u);
 	uint32_t my_steps = 0;
 	const uint32_t stride = gridDim.x * blockDim.x;
 	// Whole waves take consecutive particles and step in lockstep: the walk
@@ -455,7 +457,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 			}
 			float hd;
 			const int hit = propagate_wave<!MULTI>(W.g, walk, s.p.pos, nd, distance, s.last_tri, s.mat, hd,
-				path, 256);
+				path, 256, owner_at);
 			if (walk) {
 				int ev;
 				step_move(s, nd, distance, type, hit, hd, ev, pinfo);
@@ -621,6 +623,7 @@ __global__ __launch_bounds__(256) void k_query(Geometry g, const float* pos, con
 	float* out_dist, uint32_t n)
 {
 	__shared__ int s_path[PATH_LEVELS * 256];
+	__shared__ int s_owner[256];
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	const bool valid = i < n;
 	F3 p = { 0.0f, 0.0f, 0.0f }, d = { 1.0f, 0.0f, 0.0f };
@@ -635,7 +638,8 @@ __global__ __launch_bounds__(256) void k_query(Geometry g, const float* pos, con
 	}
 	// the walk of the step kernel: whole waves, candidate tests shared
 	float od;
-	const int r = propagate_wave<true>(g, valid, p, d, dd, it, im, od, s_path + threadIdx.x, 256);
+	const int r = propagate_wave<true>(g, valid, p, d, dd, it, im, od, s_path + threadIdx.x, 256,
+		s_owner + (threadIdx.x & ~63u));
 	if (valid) {
 		out_tri[i] = r;
 		out_dist[i] = od;

@@ -679,13 +679,13 @@ __device__ __forceinline__ int wave_incl_sum(int v)
 //
 // Every lane of the wave must call it (lanes with walk == false take part in
 // the shared tests only); a call from a partial wave falls back to
-// `propagate` lane by lane.
+// `propagate` lane by lane.  `owner_at` is 64 ints of LDS for this wave.
 //
 // SHARED: every lane walks the same geometry.  Otherwise (lanes of different
 // scans) the candidate tests read the owning lane's triangle tables.
 template <bool SHARED>
 __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 dir, float distance,
-	int ignore_tri, int ignore_mat, float& out_dist, int* __restrict__ path, int stride)
+	int ignore_tri, int ignore_mat, float& out_dist, int* __restrict__ path, int stride, int* owner_at)
 {
 	out_dist = distance;
 	if (__ballot(1) != ~0ull) {
@@ -717,13 +717,23 @@ __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 
 		const int total = __builtin_amdgcn_readlane(incl, 63);
 		for (int base = 0; base < total; base += 64) {
 			const int p = base + lane;
-			// owner of pair p: the last lane whose pairs start at or before p
-			int o = 0;
-#pragma unroll
-			for (int st = 32; st > 0; st >>= 1) {
-				if (__shfl(excl, o + st) <= p)
-					o += st;
-			}
+			// Owner of pair p: the last lane whose pairs start at or before
+			// p.  Each lane whose pairs start inside the chunk marks that
+			// position in `owner_at`; position 0 holds the owner carried in
+			// from earlier (the last lane with excl <= base).
+			const uint64_t before = __ballot(excl <= base);
+			const int o0 = 63 - (int)__builtin_clzll(before);
+			owner_at[lane] = 0;
+			__builtin_amdgcn_wave_barrier();
+			const int e = excl - base;
+			if (n > 0 && e > 0 && e < 64)
+				owner_at[e] = lane + 1;
+			__builtin_amdgcn_wave_barrier();
+			const int mark = (lane == 0) ? o0 + 1 : owner_at[lane];
+			const uint64_t marks = __ballot(mark != 0);
+			const uint64_t upto_me = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+			const int at = 63 - (int)__builtin_clzll(marks & upto_me);
+			const int o = __shfl(mark, at) - 1;
 			const F3 oc = { __shfl(cur.x, o), __shfl(cur.y, o), __shfl(cur.z, o) };
 			const F3 odir = { __shfl(dir.x, o), __shfl(dir.y, o), __shfl(dir.z, o) };
 			const int oign_tri = __shfl(ignore_tri, o), oign_mat = __shfl(ignore_mat, o);
