@@ -367,7 +367,9 @@ __device__ __forceinline__ bool place_secondary(const StepArgs& a, Ctrl* C, cons
 // shared across it (propagate_wave): a lane whose particle has stopped still
 // tests triangles for the others.  (Lanes that took new work after every step
 // or every few steps were measured slower: the extra ballots and atomics cost
-// more than the idle lanes they saved.)
+// more than the idle lanes they saved; so did lanes that kept a stopped
+// particle until half the wave was idle and then refilled together from the
+// wave's later particles, with a smaller grid to give them some.)
 //
 // SC: also accumulate each particle's likelihood-ratio score (free-path
 // race and boundary choices) and write it with the particle.
