@@ -362,8 +362,8 @@ __device__ __forceinline__ bool place_secondary(const StepArgs& a, Ctrl* C, cons
 // Grid-stride over the input particles by whole waves; each lane steps its
 // particle up to K times, stopping early when it leaves (detected,
 // terminated, inelastic event), and writes it out once: detected record,
-// inelastic queue or survivor buffer, each kind with one ballot and one
-// atomic per wave.  The wave steps in lockstep because the octree walk is
+// inelastic queue or survivor buffer, with one ballot per kind and one
+// atomic per wave for the last two together.  The wave steps in lockstep because the octree walk is
 // shared across it (propagate_wave): a lane whose particle has stopped still
 // tests triangles for the others.  (Lanes that took new work after every step
 // or every few steps were measured slower: the extra ballots and atomics cost
@@ -536,7 +536,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	count_steps(C, my_steps);
 }
 
-// The plain variant runs at 5 waves per SIMD (96 VGPRs, a 20-byte spill):
+// The plain variant runs at 5 waves per SIMD (96 VGPRs, a 36-byte spill):
 // measured faster than 6 waves (80 VGPRs, 84-byte spill) or 7 (72 VGPRs,
 // 128-byte spill).  The scored and multi-world variants would spill, so
 // they are left free.
