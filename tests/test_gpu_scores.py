@@ -130,3 +130,44 @@ def test_lr_derivative_matches_central_difference(dev):
         print(f"param {k}: LR {lr[k]:.5f} +- {lr_se[k]:.5f}   FD {fd:.5f} +- {fd_se:.5f}")
         assert abs(lr[k] - fd) < 4 * np.hypot(lr_se[k], fd_se)
         assert abs(lr[k]) > 3 * lr_se[k]        # a derivative that is resolved
+
+
+def test_lr_derivative_on_grating_matches_central_difference(dev):
+    """The differentiable workload: the 300 eV PMMA grating on Si, a 64x64
+    raster with 244 primaries per pixel (about 1e6).  The derivative of the
+    mean SE yield with respect to the four mean-free-path scales, from the
+    scores of one replay, against central differences (h = 5 %, same seed
+    and stream ids) within 4 combined standard errors; the two larger ones
+    must be resolved."""
+    tris = grating_tris()
+    raws = [material("si"), material("pmma")]
+    pri = W.raster_scan(64, 64, 244, 300.0, -128.0, 128.0, -128.0, 128.0, 37.0, seed=3)
+    n = len(pri)
+    params = [(0, EL), (0, IN), (1, EL), (1, IN)]
+    g = A.Geometry(dev, tris)
+    d = A.Driver(dev, 1 << 21, g, [A.Material(dev, A.HostMaterial(m)) for m in raws], seed=31,
+                 steps_per_launch=8)
+    tags = np.arange(n, dtype=np.uint32)
+    d.set_score_params(params)
+    d.push(pri, tags)
+    det, sc = d.run_scores(20 * n)
+    per = _yield_per_primary(det, n, sc)
+    lr = per.mean(axis=0)
+    lr_se = per.std(axis=0, ddof=1) / np.sqrt(n)
+    d.set_score_params([])
+    h = 0.05
+    resolved = 0
+    for k, (m, kind) in enumerate(params):
+        ys = []
+        for sgn in (+1, -1):
+            key = "el_mfp_scale" if kind == EL else "in_mfp_scale"
+            d.set_material_params(m, **{key: 1.0 + sgn * h})
+            d.push(pri, tags)
+            ys.append(_yield_per_primary(d.run(20 * n), n))
+            d.set_material_params(m, el_mfp_scale=1.0, in_mfp_scale=1.0)
+        fd_per = (ys[0] - ys[1]) / (2 * h)
+        fd, fd_se = fd_per.mean(), fd_per.std(ddof=1) / np.sqrt(n)
+        print(f"material {m} kind {kind}: LR {lr[k]:.5f} +- {lr_se[k]:.5f}   FD {fd:.5f} +- {fd_se:.5f}")
+        assert abs(lr[k] - fd) < 4 * np.hypot(lr_se[k], fd_se)
+        resolved += abs(lr[k]) > 5 * lr_se[k]
+    assert resolved >= 2
