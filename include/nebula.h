@@ -304,6 +304,10 @@ void* nbl_driver_stream(nbl_driver* d);
 int nbl_driver_set_timing(nbl_driver* d, int enable);
 int nbl_driver_timing(nbl_driver* d, double* step_ms_total, uint64_t* step_launches);
 
+/* Measurement aid (no reference counterpart): free and total device memory
+ * of `dev` (hipMemGetInfo), for reporting a run's footprint. */
+int nbl_device_memory(nbl_device* dev, uint64_t* free_bytes, uint64_t* total_bytes);
+
 /* Measurement aid (no reference counterpart): copy `bytes` between two
  * scratch device buffers with a plain copy kernel (k_copy) and return its
  * time; a known-size reference point for HBM traffic counters. */

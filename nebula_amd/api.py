@@ -128,6 +128,12 @@ class Device:
         check(lib().nbl_device_open(ordinal, C.byref(self._h)))
         self.ordinal = ordinal
 
+    def memory(self):
+        """(free, total) device memory in bytes."""
+        f, t = C.c_uint64(), C.c_uint64()
+        check(lib().nbl_device_memory(self._h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
     def synchronize(self):
         check(lib().nbl_device_synchronize(self._h))
 

@@ -1999,6 +1999,20 @@ int nbl_driver_step_states(nbl_driver* d, nbl_step_state* states, int32_t* event
 	return NBL_OK;
 }
 
+int nbl_device_memory(nbl_device* dev, uint64_t* free_bytes, uint64_t* total_bytes)
+{
+	clear_error();
+	if (!dev || !free_bytes || !total_bytes)
+		return fail(NBL_ERR_ARG, "invalid memory query arguments");
+	int rc = set_device(dev);
+	if (rc) return rc;
+	size_t f = 0, t = 0;
+	HIP_OK(hipMemGetInfo(&f, &t));
+	*free_bytes = f;
+	*total_bytes = t;
+	return NBL_OK;
+}
+
 int nbl_device_copy_probe(nbl_device* dev, uint64_t bytes, float* ms)
 {
 	clear_error();

@@ -38,8 +38,13 @@ class DifferentiableScan:
     yield is differentiated with respect to `params`."""
 
     def __init__(self, driver: A.Driver, primaries: np.ndarray, params, yield_class: str = "se",
-                 sid_base: int = 0):
+                 sid_base: int = 0, chunk: int = 0):
+        """chunk > 0: track the primaries `chunk` at a time and sum the
+        tallies, so that the detected records of one chunk only are held;
+        each primary keeps its stream id (sid_base + its index), so the
+        result is the same as in one pass."""
         self.d = driver
+        self.chunk = int(chunk) if chunk and chunk > 0 else len(primaries)
         self.pri = primaries
         self.params = [(int(m), int(k)) for m, k in params]
         if yield_class not in ("se", "bse"):
@@ -52,7 +57,7 @@ class DifferentiableScan:
         self.pixel_of = py * self.nx + px
         self.n_pixels = self.nx * self.ny
         self.per_pixel = np.bincount(self.pixel_of, minlength=self.n_pixels).astype(np.float64)
-        self.cap = 20 * len(primaries) + 1024
+        self.cap = 20 * max(self.chunk, 1) + 1024
         self.tags = np.arange(len(primaries), dtype=np.uint32)
         self.last_times = {}
 
@@ -81,28 +86,38 @@ class DifferentiableScan:
         return np.stack([np.bincount(pix, weights=w[:, k], minlength=self.n_pixels) / norm
                          for k in range(w.shape[1])], axis=1)
 
+    def _chunks(self):
+        for a in range(0, len(self.pri), self.chunk):
+            b = min(a + self.chunk, len(self.pri))
+            yield a, b
+
     def forward(self, theta: torch.Tensor) -> torch.Tensor:
         """Per-pixel yield [n_pixels] (float64) at parameters theta."""
         self._apply(theta)
         t0 = time.perf_counter()
-        self.d.push(self.pri, self.tags, sid_base=self.sid_base)
-        det = self.d.run(self.cap)
+        y = np.zeros(self.n_pixels)
+        for a, b in self._chunks():
+            self.d.push(self.pri[a:b], self.tags[a:b], sid_base=self.sid_base + a)
+            y += self._tally(self.d.run(self.cap))
         self.last_times["forward_s"] = time.perf_counter() - t0
-        return torch.from_numpy(self._tally(det))
+        return torch.from_numpy(y)
 
     def jacobian(self, theta: torch.Tensor) -> torch.Tensor:
         """d yield / d theta [n_pixels, n_params] by replaying the forward
         pass with scores."""
         self._apply(theta)
         t0 = time.perf_counter()
+        J = np.zeros((self.n_pixels, len(self.params)))
         self.d.set_score_params(self.params)
         try:
-            self.d.push(self.pri, self.tags, sid_base=self.sid_base)
-            det, sc = self.d.run_scores(self.cap)
+            for a, b in self._chunks():
+                self.d.push(self.pri[a:b], self.tags[a:b], sid_base=self.sid_base + a)
+                det, sc = self.d.run_scores(self.cap)
+                J += self._tally(det, sc)
         finally:
             self.d.set_score_params([])
         self.last_times["backward_s"] = time.perf_counter() - t0
-        return torch.from_numpy(self._tally(det, sc))
+        return torch.from_numpy(J)
 
 
 class ScanYield(torch.autograd.Function):

@@ -425,6 +425,11 @@ PMMA = MaterialSpec("pmma", Z=6, atom_density=50.1, atomic_mass_u=6.675,
                     work_function=2.5, fermi=3.0, band_gap=5.6, plasmon=19.9, width=8.0,
                     phonon_loss=0.1, inner_shells=((284.0, 0.25), (532.0, 0.10)),
                     outer_shells=(11.0, 14.0, 284.0, 532.0))
+# a silicon-dioxide-like layer material (mean per atom: Z 10, 20.03 u; 2.2 g/cm3)
+SIO2 = MaterialSpec("sio2", Z=10, atom_density=66.2, atomic_mass_u=20.03,
+                    work_function=0.9, fermi=10.0, band_gap=9.0, plasmon=22.4, width=10.0,
+                    phonon_loss=0.1, inner_shells=((99.8, 0.10), (532.0, 0.20)),
+                    outer_shells=(12.0, 99.8, 150.0, 532.0))
 
 
 @dataclass
