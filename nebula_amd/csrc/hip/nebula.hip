@@ -471,14 +471,14 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 					float g = 0.0f;
 					int gmi = -1, gmo = -1;
 					const float e0 = s.p.E;
-					oc = step_event_elastic_or_boundary(W, s, rng, ev, hit, &g, &gmi, &gmo);
+					oc = step_event_elastic_or_boundary(W, s, rng, ev, hit, pinfo.logE, &g, &gmi, &gmo);
 					if (ev == EV_INTERSECT)
 						score_barrier(sc, a.map, gmi, gmo, g);
 					if (DEP_OK && a.dep)
 						deposit(a, C, ev == EV_ELASTIC && s.p.E != e0, s, e0);
 				} else {
 					const float e0 = s.p.E;
-					oc = step_event_elastic_or_boundary(W, s, rng, ev, hit);
+					oc = step_event_elastic_or_boundary(W, s, rng, ev, hit, pinfo.logE);
 					if (DEP_OK && a.dep)
 						deposit(a, C, ev == EV_ELASTIC && s.p.E != e0, s, e0);
 				}

@@ -801,27 +801,29 @@ __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 
 struct Particle { F3 pos, dir; float E; };
 
 // physics/full_penn.h:50-57 (same body in elastic.h:54-62); the mean free
-// path is multiplied by the mechanism's scale (exact when it is 1).
-__device__ __forceinline__ float mech_sample_path(const Table1& t, float scale, float E, Rng& rng,
+// path is multiplied by the mechanism's scale (exact when it is 1).  logE is
+// m_log(E), computed once per step for both mechanisms.
+__device__ __forceinline__ float mech_sample_path(const Table1& t, float scale, float logE, Rng& rng,
 	float& tau)
 {
-	const float imfp = m_exp(t1_get(t, m_log(E)));
+	const float imfp = m_exp(t1_get(t, logE));
 	tau = (1.0f / imfp) * scale;
 	return -tau * m_log(rng_unit(rng));
 }
 
 // What the derivative estimator needs from a step's free-path race: the
 // scaled mean free paths, the winning mechanism (0 none, 1 inelastic,
-// 2 elastic) and the distance travelled.
-struct PathInfo { float tau_el, tau_in, dist; int type; };
+// 2 elastic) and the distance travelled; also log(E) of the step, which
+// the elastic event uses again (E does not change in between).
+struct PathInfo { float tau_el, tau_in, dist; int type; float logE; };
 
-// physics/kieft/elastic.h:67-115
-__device__ inline void elastic_execute(const Material& m, uint32_t fl, Particle& p, Rng& rng)
+// physics/kieft/elastic.h:67-115; logE = m_log(p.E)
+__device__ inline void elastic_execute(const Material& m, uint32_t fl, Particle& p, Rng& rng, float logE)
 {
 	Particle tp = p;
 	float cos_theta, sin_theta;
 	{
-		const float x = m_log(p.E);
+		const float x = logE;
 		const float y = rng_unit(rng);
 		cos_theta = clampr(t2_get(m.el_icdf, x, y), -1.0f, 1.0f);
 		sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
@@ -1132,6 +1134,7 @@ __device__ __forceinline__ int step_plan(const World& W, const State& s, Rng& rn
 	int& type, PathInfo& pinfo)
 {
 	pinfo.type = 0;
+	pinfo.logE = 0.0f;
 	type = 0;
 	if (!in_domain(W.g, s.p.pos))
 		return TERMINATED;
@@ -1144,9 +1147,11 @@ __device__ __forceinline__ int step_plan(const World& W, const State& s, Rng& rn
 		// core/scatter_list.h:101-107 with the reverse visiting order of
 		// common/tuple.h:114-124: elastic first, strictly shorter wins.
 		float dist = __builtin_inff();
-		const float pe = mech_sample_path(m.el_imfp, m.el_scale, s.p.E, rng, tau_el);
+		const float logE = m_log(s.p.E);
+		pinfo.logE = logE;
+		const float pe = mech_sample_path(m.el_imfp, m.el_scale, logE, rng, tau_el);
 		if (pe < dist) { type = 2; dist = pe; }
-		const float pi = mech_sample_path(m.in_imfp, m.in_scale, s.p.E, rng, tau_in);
+		const float pi = mech_sample_path(m.in_imfp, m.in_scale, logE, rng, tau_in);
 		if (pi < dist) { type = 1; dist = pi; }
 		distance = dist;
 	}
@@ -1209,7 +1214,7 @@ __device__ inline int step_event(const World& W, State& s, Rng& rng, int event, 
 	s.last_tri = -1;
 	const Material& m = W.mats[s.mat];
 	if (event == EV_ELASTIC) {
-		elastic_execute(m, W.flags, s.p, rng);
+		elastic_execute(m, W.flags, s.p, rng, m_log(s.p.E));
 		return ALIVE;
 	}
 	se_made = (m.inel_model == 0) ? penn_execute(m, W.flags, s.p, rng, se)
@@ -1220,9 +1225,10 @@ __device__ inline int step_event(const World& W, State& s, Rng& rng, int event, 
 // The events of a step other than inelastic scattering: boundary crossing
 // and elastic scattering (and nothing for EV_NONE).  The tracking kernel
 // sends inelastic events to their own kernel, so its code holds neither the
-// inelastic models nor their register pressure.
+// inelastic models nor their register pressure.  logE is m_log(s.p.E)
+// from the step's path sampling.
 __device__ inline int step_event_elastic_or_boundary(const World& W, State& s, Rng& rng, int event,
-	int hit, float* g = nullptr, int* g_mi = nullptr, int* g_mo = nullptr)
+	int hit, float logE, float* g = nullptr, int* g_mi = nullptr, int* g_mo = nullptr)
 {
 	if (event == EV_INTERSECT) {
 		const Tri tri = load_tri(W.g.tris, hit);
@@ -1230,7 +1236,7 @@ __device__ inline int step_event_elastic_or_boundary(const World& W, State& s, R
 	}
 	if (event == EV_ELASTIC) {
 		s.last_tri = -1;
-		elastic_execute(W.mats[s.mat], W.flags, s.p, rng);
+		elastic_execute(W.mats[s.mat], W.flags, s.p, rng, logE);
 	}
 	return ALIVE;
 }
