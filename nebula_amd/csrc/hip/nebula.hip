@@ -72,7 +72,6 @@ struct Ctrl {
 	uint32_t target;        // particles in flight to aim for
 	uint32_t spill_n;       // secondaries parked on the spill stack
 	uint32_t spill_take, spill_src;  // refill from the spill stack
-	uint32_t work_ctr;      // unused
 	uint64_t refill_src;
 	uint64_t queue_n, queue_next;
 	// Counters that waves of the step kernels bump atomically.  Each sits on
@@ -267,7 +266,6 @@ __global__ void k_ctrl(Ctrl* c, uint32_t slots, uint32_t spill_cap)
 	c->se_limit = slots - (c->n_surv + c->n_se);
 	c->step_out = 0;
 	c->inel_out = 0;
-	c->work_ctr = 0;
 	c->spill_ctr = c->spill_n;
 	c->stepped = 1;
 }
