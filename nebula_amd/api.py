@@ -397,6 +397,8 @@ class Driver:
                                       physics, energy_threshold, seed, steps_per_launch,
                                       C.byref(self._h)))
         self.seed = seed
+        # launch configuration (the library's defaults until set_launch / autotune)
+        self.launch = {"steps_per_launch": steps_per_launch, "blocks_per_cu": 32, "target": capacity}
 
     def push(self, particles: np.ndarray, tags=None, sid_base: int = 0, world: int = 0) -> int:
         p = np.ascontiguousarray(to_particles(particles) if particles.dtype != PART_DTYPE else particles)
@@ -440,6 +442,7 @@ class Driver:
 
     def set_launch(self, steps_per_launch: int, blocks_per_cu: int, target: int) -> None:
         check(lib().nbl_driver_set_launch(self._h, steps_per_launch, blocks_per_cu, target))
+        self.launch = {"steps_per_launch": steps_per_launch, "blocks_per_cu": blocks_per_cu, "target": target}
 
     def set_graph(self, enable: bool) -> None:
         check(lib().nbl_driver_set_graph(self._h, int(bool(enable))))
@@ -449,7 +452,9 @@ class Driver:
         the best one stays set."""
         r = _lib.TuneResult()
         check(lib().nbl_driver_autotune(self._h, iters_per_trial, C.byref(r)))
-        return {k: getattr(r, k) for k, _ in r._fields_}
+        out = {k: getattr(r, k) for k, _ in r._fields_}
+        self.launch = {k: out[k] for k in ("steps_per_launch", "blocks_per_cu", "target")}
+        return out
 
     def add_world(self, geometry: "Geometry", materials) -> int:
         """Another geometry with its own materials tracked by this driver;
@@ -525,12 +530,20 @@ class Driver:
         return out[: n.value], sc[: n.value]
 
     def run_tally(self, tag_bin: np.ndarray, n_bins: int, frame: int = 32, se_cutoff: float = 50.0,
-                  progress_s: float = 0.0):
+                  progress_s: float = 0.0, adapt_k: bool = False):
         """Track everything pushed to the end without keeping the records:
         counts[bin, 0] = detected electrons below `se_cutoff` eV (SE) and
         counts[bin, 1] = the others (BSE), binned by tag_bin[tag].  The
         detected buffer is drained after every `frame` iterations; with
-        progress_s > 0 a progress line goes to stderr that often."""
+        progress_s > 0 a progress line goes to stderr that often.
+
+        adapt_k: follow the best steps per launch as the run goes on.  The
+        best value drifts (short launches win while the buffer is dense,
+        long ones once few particles are left), so every fourth frame
+        probes a neighbouring value in {4, 8, 16, 32}, and the run moves to
+        it when its particle-steps per second beat the mean of the frames
+        on either side by more than 3 %.  Trajectories do not depend on the
+        launch configuration, so neither do the counts."""
         import sys
         import time
         t_last = time.perf_counter()
@@ -538,9 +551,44 @@ class Driver:
         buf = np.zeros(max(1, self.capacity), dtype=DET_DTYPE)
         n = C.c_uint32()
         last_steps, stalled = -1, 0
+        ks = (4, 8, 16, 32)
+        k0 = self.launch["steps_per_launch"]
+        ki = ks.index(k0) if k0 in ks else 1
+        frame_no, direction, probe, pending, before = 0, +1, None, None, None
+        steps_prev = self.steps() if adapt_k else 0
+        self.k_frames = {}  # frames run at each steps-per-launch value
         while True:
+            if adapt_k:
+                k = ks[probe] if probe is not None else ks[ki]
+                if k != self.launch["steps_per_launch"]:
+                    self.set_launch(k, self.launch["blocks_per_cu"], self.launch["target"])
+                t0 = time.perf_counter()
             self.do_iteration(frame)
             running, det, queued = self.counts()
+            k_now = self.launch["steps_per_launch"]
+            self.k_frames[k_now] = self.k_frames.get(k_now, 0) + 1
+            if adapt_k:
+                dt = time.perf_counter() - t0
+                st = self.steps()
+                rate = (st - steps_prev) / dt if dt > 0 else 0.0
+                steps_prev = st
+                if probe is not None:
+                    pending, probe = (probe, rate, before), None
+                else:
+                    if pending is not None:
+                        pk, pr, rb = pending
+                        ref = rate if rb is None else 0.5 * (rb + rate)
+                        if pr > 1.03 * ref:
+                            ki = pk
+                        pending = None
+                    before = rate
+                    frame_no += 1
+                    if frame_no % 4 == 0:
+                        nxt = ki + direction
+                        if not 0 <= nxt < len(ks):
+                            direction, nxt = -direction, ki - direction
+                        direction = -direction
+                        probe = nxt
             if det:
                 check(lib().nbl_driver_drain_detected(self._h, ptr(buf), len(buf), C.byref(n)))
                 got = buf[: n.value]

@@ -32,11 +32,14 @@ def geometries(dev: A.Device, scans):
 
 
 def run_batched(dev: A.Device, scans, materials, capacity: int = 1 << 21, seed: int = 1,
-                autotune: bool = True, graph: bool = True, iters_per_trial: int = 4, geoms=None,
-                progress_s: float = 0.0):
+                autotune: bool = False, graph: bool = True, iters_per_trial: int = 4, geoms=None,
+                progress_s: float = 0.0, adapt_k: bool = True):
     """All scans in one driver.  Returns (counts [n_scans, 2] of SE / BSE
     detections, seconds of tracking, tuning result or None).  Geometry and
-    material upload and the primaries' transfer to HBM precede the clock."""
+    material upload and the primaries' transfer to HBM precede the clock.
+    adapt_k: the frame loop follows the best steps per launch as the work
+    mix changes (Driver.run_tally); autotune: a one-off measured search
+    before the run (nbl_driver_autotune), which sees only the run's start."""
     geoms = geoms or geometries(dev, scans)
     d = A.Driver(dev, capacity, geoms[0], materials, seed=seed, steps_per_launch=8)
     worlds = [0] + [d.add_world(g, materials) for g in geoms[1:]]
@@ -49,7 +52,9 @@ def run_batched(dev: A.Device, scans, materials, capacity: int = 1 << 21, seed: 
     dev.synchronize()
     t0 = time.perf_counter()
     tune = d.autotune(iters_per_trial) if autotune else None
-    counts = d.run_tally(scan_of_tag, len(scans), progress_s=progress_s)
+    counts = d.run_tally(scan_of_tag, len(scans), progress_s=progress_s, adapt_k=adapt_k)
+    if adapt_k:
+        tune = dict(tune or {}, frames_per_steps_per_launch=dict(sorted(d.k_frames.items())))
     dev.synchronize()
     t = time.perf_counter() - t0
     d.close()

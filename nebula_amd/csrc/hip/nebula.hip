@@ -1428,6 +1428,9 @@ int nbl_driver_iterate(nbl_driver* d, uint32_t n_iter)
 	return NBL_OK;
 }
 
+// a candidate launch configuration must beat the incumbent by this much
+constexpr double NBL_TUNE_MARGIN = 0.03;
+
 int nbl_driver_autotune(nbl_driver* d, uint32_t iters_per_trial, nbl_tune_result* out)
 {
 	clear_error();
@@ -1439,23 +1442,36 @@ int nbl_driver_autotune(nbl_driver* d, uint32_t iters_per_trial, nbl_tune_result
 		return rc;
 	if (d->h_ctrl->queue_n == d->h_ctrl->queue_next)
 		return fail(NBL_ERR_ARG, "autotune measures queued primaries: push the work first");
-	const bool graph = d->use_graph;
+	const bool graph = d->use_graph, timing = d->timing;
+	const double step_ms0 = d->step_ms;
+	const uint64_t launches0 = d->step_launches;
 	d->use_graph = false;
+	d->timing = true;
+	// the caller's timing totals do not include the trials
+	auto restore = [&]() {
+		d->use_graph = graph;
+		d->timing = timing;
+		d->step_ms = step_ms0;
+		d->step_launches = launches0;
+	};
 	struct Cfg { uint32_t K, bpc, target; };
 	Cfg best{ d->K, d->blocks_per_cu, d->target };
 	double best_rate = -1.0;
 	uint32_t trials = 0;
 	bool stop = false;
-	// particle-steps per second of `iters_per_trial` iterations on the queued work
+	// particle-steps per second of step-kernel time (HIP events around each
+	// k_step + k_inel pair) over `iters_per_trial` iterations of the queued
+	// work; host round trips of the short trials would otherwise favour the
+	// configurations that do more work per launch
 	auto trial = [&](const Cfg& c, double& rate) -> int {
 		int r = nbl_driver_set_launch(d, c.K, c.bpc, c.target);
 		if (r) return r;
 		if ((r = read_ctrl(d))) return r;
 		const uint64_t s0 = d->h_ctrl->steps;
-		const auto t0 = std::chrono::steady_clock::now();
+		const double ms0 = d->step_ms;
 		if ((r = nbl_driver_iterate(d, iters_per_trial))) return r;
 		if ((r = read_ctrl(d))) return r;
-		const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+		const double dt = (d->step_ms - ms0) * 1e-3;
 		rate = dt > 0 ? (double)(d->h_ctrl->steps - s0) / dt : 0.0;
 		++trials;
 		// stop before the detected buffer or the queue runs out
@@ -1466,36 +1482,35 @@ int nbl_driver_autotune(nbl_driver* d, uint32_t iters_per_trial, nbl_tune_result
 	};
 	// ramp the population up with the current configuration
 	double rate = 0;
-	if ((rc = trial(best, rate))) { d->use_graph = graph; return rc; }
-	if ((rc = trial(best, rate))) { d->use_graph = graph; return rc; }
+	if ((rc = trial(best, rate))) { restore(); return rc; }
+	if ((rc = trial(best, rate))) { restore(); return rc; }
 	best_rate = rate;
 	const uint32_t Ks[] = { 4, 8, 16, 32 };
 	const uint32_t Bs[] = { 4, 8, 16, 32 };
 	const uint32_t Ts[] = { std::max<uint32_t>(64, d->capacity / 4), std::max<uint32_t>(64, d->capacity / 2),
 		d->capacity };
-	// coordinate search: steps per launch, then grid, then particles in flight
+	// Coordinate search (steps per launch, then grid, then particles in
+	// flight).  The work mix drifts as the run goes on, so each candidate is
+	// measured right next to the incumbent and replaces it only when faster
+	// by more than NBL_TUNE_MARGIN.
 	for (int dim = 0; dim < 3 && !stop; ++dim) {
 		const int nc = dim == 2 ? 3 : 4;
-		Cfg dim_best = best;
-		double dim_rate = best_rate;
 		for (int i = 0; i < nc && !stop; ++i) {
 			Cfg c = best;
 			if (dim == 0) c.K = Ks[i]; else if (dim == 1) c.bpc = Bs[i]; else c.target = Ts[i];
 			if (c.K == best.K && c.bpc == best.bpc && c.target == best.target)
 				continue;
-			if ((rc = trial(c, rate))) { d->use_graph = graph; return rc; }
-			if (rate > dim_rate) { dim_rate = rate; dim_best = c; }
+			double rc_c = 0, rc_b = 0;
+			if ((rc = trial(c, rc_c))) { restore(); return rc; }
+			if ((rc = trial(best, rc_b))) { restore(); return rc; }
+			best_rate = rc_b;
+			if (rc_c > rc_b * (1.0 + NBL_TUNE_MARGIN)) {
+				best = c;
+				best_rate = rc_c;
+			}
 		}
-		// re-measure the incumbent: the work mix drifts while tuning
-		if (!stop && !(dim_best.K == best.K && dim_best.bpc == best.bpc && dim_best.target == best.target)) {
-			double again = 0;
-			if ((rc = trial(best, again))) { d->use_graph = graph; return rc; }
-			if (again >= dim_rate) dim_best = best, dim_rate = again;
-		}
-		best = dim_best;
-		best_rate = dim_rate;
 	}
-	d->use_graph = graph;
+	restore();
 	if ((rc = nbl_driver_set_launch(d, best.K, best.bpc, best.target)))
 		return rc;
 	out->steps_per_launch = best.K;
