@@ -779,6 +779,11 @@ struct nbl_driver {
 	std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
 	double step_ms;
 	uint64_t step_launches;
+	// pinned staging of detected records (and scores) for nbl_driver_run
+	nbl_detected* h_stage = nullptr;
+	double* h_stage_sc = nullptr;
+	size_t stage_sc_n = 0;
+	hipEvent_t stage_ev = nullptr;
 };
 
 namespace {
@@ -1100,6 +1105,9 @@ int nbl_driver_destroy(nbl_driver* d)
 	hipFree(d->inel_mem);
 	hipFree(d->ctrl);
 	hipHostFree(d->h_ctrl);
+	hipHostFree(d->h_stage);
+	hipHostFree(d->h_stage_sc);
+	if (d->stage_ev) hipEventDestroy(d->stage_ev);
 	hipFree(d->det);
 	hipFree(d->d_mats);
 	hipFree(d->q_part);
@@ -1750,55 +1758,82 @@ int nbl_driver_steps(nbl_driver* d, uint64_t* steps)
 }
 
 namespace {
+// Track everything pushed to the end.  Each frame's detected records are
+// copied on the stream into pinned staging and the counter reset, and the
+// next frame is enqueued at once; the host then moves the staged records
+// into `out` while the GPU runs that frame.
 int run(nbl_driver* d, nbl_detected* out, double* scores, uint64_t cap, uint64_t* n_detected)
 {
 	const size_t np = scores ? (size_t)d->map.np : 0;
+	if (!d->h_stage &&
+		hipHostMalloc((void**)&d->h_stage, sizeof(nbl_detected) * d->det_cap, hipHostMallocDefault) != hipSuccess) {
+		d->h_stage = nullptr;
+		return fail(NBL_ERR_ALLOC, "pinned staging for detected records");
+	}
+	if (np && d->stage_sc_n < np * d->det_cap) {
+		hipHostFree(d->h_stage_sc);
+		d->stage_sc_n = 0;
+		if (hipHostMalloc((void**)&d->h_stage_sc, sizeof(double) * np * d->det_cap, hipHostMallocDefault)
+			!= hipSuccess) {
+			d->h_stage_sc = nullptr;
+			return fail(NBL_ERR_ALLOC, "pinned staging for detected scores");
+		}
+		d->stage_sc_n = np * d->det_cap;
+	}
+	if (!d->stage_ev)
+		HIP_OK(hipEventCreateWithFlags(&d->stage_ev, hipEventDisableTiming));
 	uint64_t total = 0;
 	uint32_t frame = 4;
-	std::vector<nbl_detected> tmp;
-	std::vector<double> tmp_sc;
 	uint64_t last_steps = ~0ull;
 	int stalled = 0;
+	int rc = nbl_driver_iterate(d, frame);
+	if (rc) return rc;
 	for (;;) {
-		int rc = nbl_driver_iterate(d, frame);
-		if (rc) return rc;
 		uint32_t running = 0, det = 0;
 		uint64_t queued = 0;
 		if ((rc = nbl_driver_counts(d, &running, &det, &queued)))
 			return rc;
 		if (det) {
-			uint32_t got = 0;
-			const uint64_t room = cap > total ? cap - total : 0;
-			if (room >= det) {
-				if ((rc = drain(d, out + total, scores ? scores + total * np : nullptr, det, &got)))
-					return rc;
+			HIP_OK(hipMemcpyAsync(d->h_stage, d->det, sizeof(nbl_detected) * det, hipMemcpyDeviceToHost,
+				d->stream));
+			if (np)
+				HIP_OK(hipMemcpyAsync(d->h_stage_sc, d->det_sc, sizeof(double) * np * (size_t)det,
+					hipMemcpyDeviceToHost, d->stream));
+			hipLaunchKernelGGL(k_reset_det, dim3(1), dim3(64), 0, d->stream, d->ctrl);
+			HIP_OK(hipEventRecord(d->stage_ev, d->stream));
+		}
+		// particles waiting for a record slot still count as running
+		const bool finished = running == 0 && queued == 0;
+		if (!finished) {
+			// Every particle waiting for a secondary slot with the buffer and
+			// the spill stack full would never finish: report it instead of
+			// spinning.
+			if (d->h_ctrl->steps == last_steps && det == 0) {
+				if (++stalled >= 4)
+					return fail(NBL_ERR_CAPACITY, "no progress: particle buffer and spill stack full of "
+						"particles waiting for secondary-electron slots; raise the capacity");
 			} else {
-				tmp.resize(det);
-				tmp_sc.resize((size_t)det * std::max<size_t>(np, 1));
-				if ((rc = drain(d, tmp.data(), scores ? tmp_sc.data() : nullptr, det, &got)))
-					return rc;
-				if (room) {
-					std::memcpy(out + total, tmp.data(), sizeof(nbl_detected) * room);
-					if (scores)
-						std::memcpy(scores + total * np, tmp_sc.data(), sizeof(double) * np * room);
-				}
+				stalled = 0;
 			}
-			total += got;
+			last_steps = d->h_ctrl->steps;
+			frame = std::min<uint32_t>(frame * 2, 32);
+			if ((rc = nbl_driver_iterate(d, frame)))
+				return rc;
 		}
-		if (running == 0 && queued == 0 && d->h_ctrl->det_ctr == 0)
+		if (det) {
+			HIP_OK(hipEventSynchronize(d->stage_ev));
+			const uint64_t room = cap > total ? std::min<uint64_t>(cap - total, det) : 0;
+			if (room) {
+				std::memcpy(out + total, d->h_stage, sizeof(nbl_detected) * room);
+				if (np)
+					std::memcpy(scores + total * np, d->h_stage_sc, sizeof(double) * np * room);
+			}
+			total += det;
+		}
+		if (finished)
 			break;
-		// Every particle waiting for a secondary slot with the buffer and the
-		// spill stack full would never finish: report it instead of spinning.
-		if (d->h_ctrl->steps == last_steps && det == 0) {
-			if (++stalled >= 4)
-				return fail(NBL_ERR_CAPACITY, "no progress: particle buffer and spill stack full of "
-					"particles waiting for secondary-electron slots; raise the capacity");
-		} else {
-			stalled = 0;
-		}
-		last_steps = d->h_ctrl->steps;
-		frame = std::min<uint32_t>(frame * 2, 32);
 	}
+	HIP_OK(hipStreamSynchronize(d->stream));
 	*n_detected = total;
 	if (total > cap)
 		return fail(NBL_ERR_CAPACITY, "more detected electrons than the output array holds");
