@@ -555,23 +555,29 @@ class Driver:
         k0 = self.launch["steps_per_launch"]
         ki = ks.index(k0) if k0 in ks else 1
         frame_no, direction, probe, pending, before = 0, +1, None, None, None
-        steps_prev = self.steps() if adapt_k else 0
+        steps_prev = self.steps()
         self.k_frames = {}  # frames run at each steps-per-launch value
-        while True:
+
+        def launch_frame():
             if adapt_k:
                 k = ks[probe] if probe is not None else ks[ki]
                 if k != self.launch["steps_per_launch"]:
                     self.set_launch(k, self.launch["blocks_per_cu"], self.launch["target"])
-                t0 = time.perf_counter()
-            self.do_iteration(frame)
-            running, det, queued = self.counts()
             k_now = self.launch["steps_per_launch"]
             self.k_frames[k_now] = self.k_frames.get(k_now, 0) + 1
+            self.do_iteration(frame)
+            return time.perf_counter()
+
+        # Each frame's records are drained as soon as it ends and the next
+        # frame is enqueued before they are tallied, so the host's tally
+        # overlaps the GPU's next frame.
+        t0 = launch_frame()
+        while True:
+            running, det, queued = self.counts()
+            dt = time.perf_counter() - t0
+            st = self.steps()
             if adapt_k:
-                dt = time.perf_counter() - t0
-                st = self.steps()
                 rate = (st - steps_prev) / dt if dt > 0 else 0.0
-                steps_prev = st
                 if probe is not None:
                     pending, probe = (probe, rate, before), None
                 else:
@@ -589,23 +595,28 @@ class Driver:
                             direction, nxt = -direction, ki - direction
                         direction = -direction
                         probe = nxt
+            steps_prev = st
+            got = None
             if det:
                 check(lib().nbl_driver_drain_detected(self._h, ptr(buf), len(buf), C.byref(n)))
                 got = buf[: n.value]
+            finished = running == 0 and queued == 0
+            if not finished:
+                stalled = stalled + 1 if (st == last_steps and det == 0) else 0
+                last_steps = st
+                if stalled >= 4:
+                    raise NebulaError(_lib.NBL_ERR_CAPACITY, "no progress: raise the capacity")
+                t0 = launch_frame()
+            if got is not None:
                 b = tag_bin[got["tag"]]
                 se = got["E"] < se_cutoff
                 counts[:, 0] += np.bincount(b[se], minlength=n_bins)
                 counts[:, 1] += np.bincount(b[~se], minlength=n_bins)
-            if running == 0 and queued == 0 and det == 0:
+            if finished:
                 return counts
             if progress_s > 0 and time.perf_counter() - t_last > progress_s:
                 t_last = time.perf_counter()
                 print(f"  tracking: {running} in flight, {queued} queued", file=sys.stderr, flush=True)
-            st = self.steps()
-            stalled = stalled + 1 if (st == last_steps and det == 0) else 0
-            last_steps = st
-            if stalled >= 4:
-                raise NebulaError(_lib.NBL_ERR_CAPACITY, "no progress: raise the capacity")
 
     def steps(self) -> int:
         s = C.c_uint64()
