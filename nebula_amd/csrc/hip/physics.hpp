@@ -515,14 +515,21 @@ __device__ __forceinline__ int cell_find(const Geometry& g, F3 cur, uint64_t& lo
 {
 	const int* __restrict__ od = g.od;
 	const F3 hs = g.halfsize;
-	ax = g.center.x; ay = g.center.y; az = g.center.z; aw = 1.0f;
+	// The reference adds aw * hs * (bit - 0.5) per level.  The product with
+	// +-0.5 is exact, and scaling by a power of two commutes with rounding
+	// (aw >= 2^-21 and hs >= 1 nm keep every product far from underflow), so
+	// it equals +-(hw * hs) with hw = aw / 2: the same bits in three
+	// operations per axis instead of six.
+	ax = g.center.x; ay = g.center.y; az = g.center.z;
+	float hw = 0.5f;
 	const int clz = __clzll((long long)location);
 	for (int i = 60 - clz; i >= 0; i -= 3) {
 		const unsigned oct = (unsigned)(location >> i) & 7u;
-		ax += aw * hs.x * ((float)(oct & 1u) - 0.5f);
-		ay += aw * hs.y * ((float)((oct & 2u) >> 1) - 0.5f);
-		az += aw * hs.z * ((float)((oct & 4u) >> 2) - 0.5f);
-		aw *= 0.5f;
+		const float hx = hw * hs.x, hy = hw * hs.y, hz = hw * hs.z;
+		ax += (oct & 1u) ? hx : -hx;
+		ay += (oct & 2u) ? hy : -hy;
+		az += (oct & 4u) ? hz : -hz;
+		hw *= 0.5f;
 	}
 	// levels 1..L-1 are unchanged ancestors; level L holds the new octant
 	int lev = (63 - clz) / 3;
@@ -540,11 +547,13 @@ __device__ __forceinline__ int cell_find(const Geometry& g, F3 cur, uint64_t& lo
 		location = (location << 3) | oct;
 		index = od[index + (int)oct];
 		path[++lev * stride] = index;
-		ax += aw * hs.x * ((float)(oct & 1u) - 0.5f);
-		ay += aw * hs.y * ((float)((oct & 2u) >> 1) - 0.5f);
-		az += aw * hs.z * ((float)((oct & 4u) >> 2) - 0.5f);
-		aw *= 0.5f;
+		const float hx = hw * hs.x, hy = hw * hs.y, hz = hw * hs.z;
+		ax += (oct & 1u) ? hx : -hx;
+		ay += (oct & 2u) ? hy : -hy;
+		az += (oct & 4u) ? hz : -hz;
+		hw *= 0.5f;
 	}
+	aw = 2.0f * hw;
 	return -index;
 }
 
