@@ -254,9 +254,10 @@ typedef struct {
 int nbl_driver_set_deposits(nbl_driver* d, int enable);
 int nbl_driver_drain_deposits(nbl_driver* d, nbl_deposit* out, uint64_t cap, uint64_t* n);
 
-/* Launch configuration: steps each particle takes per k_step launch,
- * k_step blocks (256 threads) per compute unit, and particles in flight to
- * aim for (<= capacity; the reference's batch sizing, gpu_main.cu:167-175). */
+/* Launch configuration: steps each particle takes per k_step launch (1 to
+ * 1024), k_step blocks (256 threads) per compute unit (1 to 128; default 64,
+ * and never more than the capacity needs), and particles in flight to aim
+ * for (<= capacity; the reference's batch sizing, gpu_main.cu:167-175). */
 int nbl_driver_set_launch(nbl_driver* d, uint32_t steps_per_launch, uint32_t blocks_per_cu,
 	uint32_t target);
 
@@ -266,7 +267,7 @@ int nbl_driver_set_graph(nbl_driver* d, int enable);
 
 /* The prescan autotune of gpu_main.cu:136-175 made a measured search:
  * coordinate search over steps per launch {4,8,16,32}, blocks per compute
- * unit {4,8,16,32} and particles in flight {capacity/4, /2, 1}, each trial
+ * unit {8,16,32,64} and particles in flight {capacity/4, /2, 1}, each trial
  * `iters_per_trial` iterations of the queued primaries (real work: their
  * detected electrons stay in the buffer for the next drain).  Leaves the
  * fastest configuration (particle-steps/s) set on the driver. */

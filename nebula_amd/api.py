@@ -398,7 +398,7 @@ class Driver:
                                       C.byref(self._h)))
         self.seed = seed
         # launch configuration (the library's defaults until set_launch / autotune)
-        self.launch = {"steps_per_launch": steps_per_launch, "blocks_per_cu": 32, "target": capacity}
+        self.launch = {"steps_per_launch": steps_per_launch, "blocks_per_cu": 64, "target": capacity}
 
     def push(self, particles: np.ndarray, tags=None, sid_base: int = 0, world: int = 0) -> int:
         p = np.ascontiguousarray(to_particles(particles) if particles.dtype != PART_DTYPE else particles)

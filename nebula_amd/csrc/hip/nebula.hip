@@ -766,7 +766,7 @@ struct nbl_driver {
 	double* sc_mem[4];      // score arrays of buf[0], buf[1], spill, inel
 	double* det_sc;
 	std::vector<Material> h_mats;   // host copy of d_mats
-	uint32_t blocks_per_cu = 32;    // k_step grid: blocks of 256 per compute unit
+	uint32_t blocks_per_cu = 64;    // k_step grid: blocks of 256 per compute unit
 	uint32_t target = 0;            // particles in flight to aim for
 	nbl_deposit* dep = nullptr;     // energy-deposit records (enabled when non-null)
 	uint64_t dep_cap = 0;
@@ -1494,7 +1494,7 @@ int nbl_driver_autotune(nbl_driver* d, uint32_t iters_per_trial, nbl_tune_result
 	if ((rc = trial(best, rate))) { restore(); return rc; }
 	best_rate = rate;
 	const uint32_t Ks[] = { 4, 8, 16, 32 };
-	const uint32_t Bs[] = { 4, 8, 16, 32 };
+	const uint32_t Bs[] = { 8, 16, 32, 64 };
 	const uint32_t Ts[] = { std::max<uint32_t>(64, d->capacity / 4), std::max<uint32_t>(64, d->capacity / 2),
 		d->capacity };
 	// Coordinate search (steps per launch, then grid, then particles in
@@ -1600,8 +1600,8 @@ int nbl_driver_set_launch(nbl_driver* d, uint32_t steps_per_launch, uint32_t blo
 		return fail(NBL_ERR_ARG, "null driver");
 	if (steps_per_launch < 1 || steps_per_launch > 1024)
 		return fail(NBL_ERR_ARG, "steps_per_launch must be in [1, 1024]");
-	if (blocks_per_cu < 1 || blocks_per_cu > 32)
-		return fail(NBL_ERR_ARG, "blocks_per_cu must be in [1, 32]");
+	if (blocks_per_cu < 1 || blocks_per_cu > 128)
+		return fail(NBL_ERR_ARG, "blocks_per_cu must be in [1, 128]");
 	if (target < 1 || target > d->capacity)
 		return fail(NBL_ERR_ARG, "target must be in [1, capacity]");
 	if (d->dep && steps_per_launch + 1 > d->dep_per_particle)
