@@ -233,6 +233,52 @@ def bench_grad(dev):
             "d_mean_se_yield": [float(v) for v in total]}
 
 
+def bench_grad_reference(dev, replicas=100):
+    """The 1e6-primary gradient of the differentiable scan against a
+    reference gradient from `replicas` times as many primaries.  The 64x64
+    grating raster at 244 primaries per pixel is tracked 1 + replicas times
+    with fresh spot positions (raster seed 1 + r) and fresh stream ids
+    (r << 40): replica 0 is the estimate that the gradient bench reports,
+    replicas 1..R together are the reference (independent of replica 0), and
+    the spread of the replicas is the standard error of one estimate."""
+    from nebula_amd import grad as G
+    mesh, _, mats = workload(64, 64, 244)
+    g = A.Geometry(dev, mesh)
+    gm = [A.Material(dev, A.HostMaterial(m)) for m in mats]
+    d = A.Driver(dev, 1 << 22, g, gm, seed=1, steps_per_launch=5)
+    params = [(0, 0), (0, 1), (1, 0), (1, 1), (1, 2)]
+    names = ["Si el MFP scale", "Si in MFP scale", "PMMA el MFP scale", "PMMA in MFP scale", "PMMA barrier"]
+    est = np.zeros((replicas + 1, len(params)))
+    j_ref = None
+    t0 = time.perf_counter()
+    n = 0
+    for r in range(replicas + 1):
+        pri = W.raster_scan(64, 64, 244, 300.0, -128.0, 128.0, -128.0, 128.0, 37.0, spot_sigma=1.0, seed=1 + r)
+        scan = G.DifferentiableScan(d, pri, params, sid_base=r << 40)
+        J = scan.jacobian(scan.theta0()).numpy()
+        est[r] = (J * scan.per_pixel[:, None]).sum(axis=0) / len(pri)
+        if r == 0:
+            j0, n = J, len(pri)
+        else:
+            j_ref = J if j_ref is None else j_ref + J
+        if r % 10 == 0:
+            print(f"grad reference: {r + 1} of {replicas + 1} replays, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+    ref = est[1:].mean(axis=0)
+    sd = est[1:].std(axis=0, ddof=1)
+    j_ref /= replicas
+    out = []
+    for k, name in enumerate(names):
+        out.append({"param": name, "estimate": float(est[0, k]), "reference": float(ref[k]),
+                    "rel_diff": float(abs(est[0, k] - ref[k]) / abs(ref[k])),
+                    "estimate_rel_se": float(sd[k] / abs(ref[k])),
+                    "reference_rel_se": float(sd[k] / np.sqrt(replicas) / abs(ref[k]))})
+    return {"workload": f"grating 64x64 px x 244 primaries at 300 eV ({n} primaries) vs {replicas} fresh "
+                        f"replicas ({replicas * n} primaries)",
+            "seconds": time.perf_counter() - t0, "d_mean_se_yield": out,
+            "per_pixel_rel_l2": float(np.linalg.norm(j0 - j_ref) / np.linalg.norm(j_ref))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +302,9 @@ def main():
     ap.add_argument("--large", type=float, default=0.0,
                     help="run only the large configuration (2e6-triangle contact holes, 1e8 x 1 keV "
                          "primaries times this scale, 7-parameter gradient) and print its JSON line")
+    ap.add_argument("--grad-reference", type=int, default=0,
+                    help="run only the gradient check against a reference gradient from this many "
+                         "times the primaries (fresh replicas of the 1e6-primary scan); print its JSON line")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "step_pair_traffic.json"),
                     help="measured HBM bytes per k_step + k_inel launch pair (from rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -269,6 +318,14 @@ def main():
                           "value": r["primaries_per_s"], "unit": "primaries/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "f32", "data": "synthetic", "large": r}),
               flush=True)
+        return
+    if args.grad_reference > 0:
+        r = bench_grad_reference(A.Device(0), args.grad_reference)
+        print(json.dumps({"metric": "gradient of the 1e6-primary scan vs a reference gradient from "
+                                    f"{args.grad_reference}x the primaries",
+                          "value": max(v["rel_diff"] for v in r["d_mean_se_yield"]), "unit": "relative difference",
+                          "n_gpus": 1, "higher_is_better": False, "dtype": "f32 tracking, f64 scores",
+                          "data": "synthetic", "grad_reference": r}), flush=True)
         return
 
     mesh, pri, mats = workload(args.pixels, args.pixels, args.per_pixel)
