@@ -233,14 +233,20 @@ def bench_grad(dev):
             "d_mean_se_yield": [float(v) for v in total]}
 
 
-def bench_grad_reference(dev, replicas=100):
+def bench_grad_reference(dev, replicas=100, fd_h=1e-3):
     """The 1e6-primary gradient of the differentiable scan against a
     reference gradient from `replicas` times as many primaries.  The 64x64
     grating raster at 244 primaries per pixel is tracked 1 + replicas times
     with fresh spot positions (raster seed 1 + r) and fresh stream ids
     (r << 40): replica 0 is the estimate that the gradient bench reports,
     replicas 1..R together are the reference (independent of replica 0), and
-    the spread of the replicas is the standard error of one estimate."""
+    the spread of the replicas is the standard error of one estimate.
+
+    fd_h > 0 adds central finite differences with step fd_h and common
+    random numbers (the same primaries and stream ids at theta +- h) on the
+    reference replicas.  The barrier's finite difference also holds its
+    deterministic effects (energy shift, refraction), which its
+    likelihood-ratio derivative leaves out by construction."""
     from nebula_amd import grad as G
     mesh, _, mats = workload(64, 64, 244)
     g = A.Geometry(dev, mesh)
@@ -249,14 +255,25 @@ def bench_grad_reference(dev, replicas=100):
     params = [(0, 0), (0, 1), (1, 0), (1, 1), (1, 2)]
     names = ["Si el MFP scale", "Si in MFP scale", "PMMA el MFP scale", "PMMA in MFP scale", "PMMA barrier"]
     est = np.zeros((replicas + 1, len(params)))
+    fd = np.zeros((replicas + 1, len(params)))
     j_ref = None
     t0 = time.perf_counter()
     n = 0
+    theta = None
     for r in range(replicas + 1):
         pri = W.raster_scan(64, 64, 244, 300.0, -128.0, 128.0, -128.0, 128.0, 37.0, spot_sigma=1.0, seed=1 + r)
         scan = G.DifferentiableScan(d, pri, params, sid_base=r << 40)
-        J = scan.jacobian(scan.theta0()).numpy()
+        if theta is None:
+            theta = scan.theta0()     # the finite differences leave the driver at theta - h
+        J = scan.jacobian(theta).numpy()
         est[r] = (J * scan.per_pixel[:, None]).sum(axis=0) / len(pri)
+        for k in range(len(params) if fd_h > 0 and r > 0 else 0):
+            ys = []
+            for sgn in (1.0, -1.0):
+                th = theta.clone()
+                th[k] += sgn * fd_h
+                ys.append((scan.forward(th).numpy() * scan.per_pixel).sum() / len(pri))
+            fd[r, k] = (ys[0] - ys[1]) / (2.0 * fd_h)
         if r == 0:
             j0, n = J, len(pri)
         else:
@@ -266,6 +283,7 @@ def bench_grad_reference(dev, replicas=100):
                   file=sys.stderr, flush=True)
     ref = est[1:].mean(axis=0)
     sd = est[1:].std(axis=0, ddof=1)
+    fd_mean, fd_se = fd[1:].mean(axis=0), fd[1:].std(axis=0, ddof=1) / np.sqrt(replicas)
     j_ref /= replicas
     out = []
     for k, name in enumerate(names):
@@ -273,9 +291,12 @@ def bench_grad_reference(dev, replicas=100):
                     "rel_diff": float(abs(est[0, k] - ref[k]) / abs(ref[k])),
                     "estimate_rel_se": float(sd[k] / abs(ref[k])),
                     "reference_rel_se": float(sd[k] / np.sqrt(replicas) / abs(ref[k]))})
+        if fd_h > 0:
+            out[-1].update({"fd": float(fd_mean[k]), "fd_se": float(fd_se[k]),
+                            "lr_minus_fd_in_se": float((ref[k] - fd_mean[k]) / np.hypot(fd_se[k], sd[k] / np.sqrt(replicas)))})
     return {"workload": f"grating 64x64 px x 244 primaries at 300 eV ({n} primaries) vs {replicas} fresh "
                         f"replicas ({replicas * n} primaries)",
-            "seconds": time.perf_counter() - t0, "d_mean_se_yield": out,
+            "seconds": time.perf_counter() - t0, "fd_h": fd_h, "d_mean_se_yield": out,
             "per_pixel_rel_l2": float(np.linalg.norm(j0 - j_ref) / np.linalg.norm(j_ref))}
 
 
