@@ -79,7 +79,7 @@ struct Ctrl {
 	// and the two a wave always needs together share one 64-bit word, so
 	// that one atomic returns both bases.
 	alignas(128) uint64_t step_out;   // k_step: survivors (low 32) | inelastic queue (high 32)
-	alignas(128) uint64_t inel_out;   // k_inel: secondaries (low 32) | parents kept (high 32)
+	alignas(128) uint32_t se_ctr;     // k_inel: secondaries made (one atomic per block)
 	alignas(128) uint32_t det_ctr;    // detected records written (may overshoot det_cap)
 	alignas(128) uint32_t spill_ctr;  // spill stack top during a k_step (may overshoot)
 	alignas(128) uint32_t dep_ctr;    // energy-deposit records written (may overshoot dep_cap)
@@ -89,9 +89,10 @@ struct Ctrl {
 // Counts held in the packed words.
 __host__ __device__ inline uint32_t lo32(uint64_t v) { return (uint32_t)v; }
 __host__ __device__ inline uint32_t hi32(uint64_t v) { return (uint32_t)(v >> 32); }
-// survivors written by the last k_step + k_inel pair, secondaries requested
-__host__ __device__ inline uint32_t out_surv(const Ctrl& c) { return lo32(c.step_out) + hi32(c.inel_out); }
-__host__ __device__ inline uint32_t out_se(const Ctrl& c) { return lo32(c.inel_out); }
+// survivors written by the last k_step + k_inel pair (k_inel keeps every
+// queued particle), secondaries requested
+__host__ __device__ inline uint32_t out_surv(const Ctrl& c) { return lo32(c.step_out) + hi32(c.step_out); }
+__host__ __device__ inline uint32_t out_se(const Ctrl& c) { return c.se_ctr; }
 
 struct StepArgs {
 	World W;
@@ -265,7 +266,7 @@ __global__ void k_ctrl(Ctrl* c, uint32_t slots, uint32_t spill_cap)
 	c->n_surv += take + m;
 	c->se_limit = slots - (c->n_surv + c->n_se);
 	c->step_out = 0;
-	c->inel_out = 0;
+	c->se_ctr = 0;
 	c->spill_ctr = c->spill_n;
 	c->stepped = 1;
 }
@@ -557,50 +558,76 @@ void k_step<false, false>(StepArgs a)
 // forget the last triangle, run the inelastic model from draw 2 of the step's
 // stream, place the secondary, and keep the particle for the next launch.
 // The inelastic models carry no parameter of the score.
+//
+// Every queued particle comes back as a survivor, so queue entry i goes to
+// survivor slot surv0 + i with no atomic.  Secondary slots are counted per
+// block (ballot per wave, the four counts in LDS, one atomic per block):
+// with one atomic per wave, the waves' atomics on one counter serialised in
+// L2 and this kernel took 7 % more of the iteration.  The block loop runs
+// whole blocks, so every thread reaches each barrier.
 template <bool SC, bool MULTI>
 __global__ __launch_bounds__(256) void k_inel(StepArgs a)
 {
 	Ctrl* C = a.ctrl;
 	const uint32_t bi = C->buf_in;
 	const Bufs out = bi ? a.buf[0] : a.buf[1];
-	// parents go on after k_step's survivors
 	const uint32_t n = hi32(C->step_out), se_limit = C->se_limit, surv0 = lo32(C->step_out);
 	const int np = a.map.np;
 	World Wl = a.W;
 	const World& W = MULTI ? Wl : a.W;
 	const uint32_t stride = gridDim.x * blockDim.x;
-	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-		State s = load_state(a.inel, i);
-		if (MULTI) {
-			const WorldDesc& wd = a.worlds[s.flags >> FL_WORLD_SHIFT];
-			Wl.mats = wd.mats;
-			Wl.nmats = wd.nmats;
-		}
+	__shared__ uint32_t s_cnt[4];
+	__shared__ uint32_t s_base;
+	const uint32_t wave = threadIdx.x >> 6;
+	for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
+		const uint32_t i = b0 + threadIdx.x;
+		const bool valid = i < n;
+		State s = {}, pre = {};
 		Score sc;
-		if (SC) load_score(a.inel, i, np, sc);
-		const State pre = s;
-		Rng rng;
-		rng_start(rng, W.key0, W.key1, s.sid, s.step);
-		rng_seek(rng, 2);
-		Particle se;
+		Particle se = {};
 		bool made = false;
-		step_event(W, s, rng, EV_INELASTIC, -1, se, made);
-		bool done = true;
-		uint32_t sslot, oslot;
-		wave_alloc2(&C->inel_out, made, true, sslot, oslot);
-		oslot += surv0;
-		if (place_secondary<SC>(a, C, out, se_limit, made, sslot, se, s, sc)) {
-			s.step++;
-		} else {
-			// no room anywhere: retry the event from the same draws later
-			s = pre;
-			s.flags |= FL_PEND_INEL;
-			done = false;
+		if (valid) {
+			s = load_state(a.inel, i);
+			if (MULTI) {
+				const WorldDesc& wd = a.worlds[s.flags >> FL_WORLD_SHIFT];
+				Wl.mats = wd.mats;
+				Wl.nmats = wd.nmats;
+			}
+			if (SC) load_score(a.inel, i, np, sc);
+			pre = s;
+			Rng rng;
+			rng_start(rng, W.key0, W.key1, s.sid, s.step);
+			rng_seek(rng, 2);
+			step_event(W, s, rng, EV_INELASTIC, -1, se, made);
 		}
-		if ((SC || MULTI) && a.dep)
-			deposit(a, C, done && s.p.E != pre.p.E, s, pre.p.E);
-		store_state(out, oslot, s);
-		if (SC) store_score(out, oslot, np, sc);
+		const uint64_t m = __ballot(made);
+		if (__lane_id() == 0)
+			s_cnt[wave] = (uint32_t)__popcll(m);
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			const uint32_t tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+			s_base = tot ? atomicAdd(&C->se_ctr, tot) : 0u;
+		}
+		__syncthreads();
+		uint32_t sslot = s_base + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+		for (uint32_t w = 0; w < wave; ++w)
+			sslot += s_cnt[w];
+		__syncthreads();   // s_cnt and s_base are rewritten by the next round
+		if (valid) {
+			bool done = true;
+			if (place_secondary<SC>(a, C, out, se_limit, made, sslot, se, s, sc)) {
+				s.step++;
+			} else {
+				// no room anywhere: retry the event from the same draws later
+				s = pre;
+				s.flags |= FL_PEND_INEL;
+				done = false;
+			}
+			if ((SC || MULTI) && a.dep)
+				deposit(a, C, done && s.p.E != pre.p.E, s, pre.p.E);
+			store_state(out, surv0 + i, s);
+			if (SC) store_score(out, surv0 + i, np, sc);
+		}
 	}
 }
 
