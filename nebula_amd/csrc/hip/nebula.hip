@@ -535,7 +535,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	count_steps(C, my_steps);
 }
 
-// The plain variant runs at 5 waves per SIMD (96 VGPRs, a 36-byte spill):
+// The plain variant runs at 5 waves per SIMD (96 VGPRs, a 40-byte spill):
 // measured faster than 6 waves (80 VGPRs, 84-byte spill) or 7 (72 VGPRs,
 // 128-byte spill).  The scored and multi-world variants would spill, so
 // they are left free.
