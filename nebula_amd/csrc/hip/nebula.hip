@@ -6,8 +6,9 @@
 // survivors of the previous launch occupy [0, n_surv) and secondary electrons
 // [slots - n_se, slots).  A launch of k_step moves every particle through up
 // to K steps (init + event each), then appends it to the other buffer:
-// survivors from the front, new secondaries from the back, each slot taken by
-// a wavefront ballot + prefix popcount + one atomic per wave.  Dead particles
+// survivors from the front (a wavefront ballot + prefix popcount + one
+// atomic per wave), new secondaries from the back (made by k_inel, one
+// atomic per block).  Dead particles
 // are simply not copied, so the next launch reads a dense range.  A
 // one-thread control kernel adopts the output counts and plans the refill of
 // primaries from a device-resident queue; all sizes live in device memory, so
