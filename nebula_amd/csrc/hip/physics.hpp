@@ -460,6 +460,10 @@ __device__ __forceinline__ Tri load_tri(const float4* t, int i)
 struct Geometry {
 	const int* od;
 	const float4* tris;
+	// Screening record per triangle, two float4: (r0, |e1|_1 |e2|_1) and
+	// (e1 x e2 as the triangle test computes it, material_in | material_out
+	// as two int16 in the bits of w).
+	const float4* scr;
 	const int* cnt;  // cnt[i]: triangles listed from od[i] to the list's end
 	F3 center, halfsize;
 	float max_extent;
@@ -588,6 +592,53 @@ __device__ __forceinline__ float cell_candidate(const Geometry& g, int ti, F3 cu
 	return intersect_ray(tri, cur, dir);
 }
 
+// First stage of the wave-wide walk's candidate test, on the 32-byte
+// screening record: the material test of cell_candidate (the same e1 x e2,
+// computed on the host with the same operations, so the same result) and a
+// plane test that drops
+// a pair only when its intersection distance cannot matter.  For the pair's
+// float inputs (s = cur - r0, dir, e1, e2), Moller-Trumbore's t is
+// t* = -(s . n) / (dir . n) with n = e1 x e2 in exact arithmetic, and
+// f(x) = s . n + x (dir . n) is the plane value at distance x along the ray.
+// The pair is dropped when f has one sign, by more than
+// tol = 1e-3 (|s| + L) |e1||e2|, at both x = -m and x = L with
+// m = 1e-2 (1 + |s|) and L = 1.01 to_go + m: then t* < -m or t* > L.
+// (Lengths here are 1-norms, which bound the 2-norms from above: every
+// margin only grows.)
+// Rounding moves f by at most about 1e-5 (|s| + L) |e1||e2| (a few unit
+// roundoffs times the terms' magnitudes), far inside tol.  Pairs whose ray
+// is within |dir . n| < 1e-2 |e1||e2| of the plane always pass: elsewhere
+// the computed determinant and numerator of intersect_ray are within
+// 1e-4 |t*| + 1e-4 |s| of their exact quotient, so a dropped pair has
+// computed t < 0 (cannot hit) or t > to_go + EPS.  Such a far result cannot
+// change the running-minimum fold: taking it needs isect >= t - EPS > to_go,
+// and from there the owner ends without a hit whatever it takes next (a
+// candidate below to_go is taken in both folds and makes them equal again).
+// Pairs that pass are tested in full (intersect_ray) in the second stage.
+__device__ __forceinline__ float norm1(F3 a) { return fabsf(a.x) + fabsf(a.y) + fabsf(a.z); }
+__device__ __forceinline__ bool cell_may_hit(const Geometry& g, int ti, F3 cur, F3 dir, int ignore_mat,
+	float to_go)
+{
+	const float4 a = g.scr[2 * ti], b = g.scr[2 * ti + 1];
+	const F3 nrm = { b.x, b.y, b.z };
+	const float dn = dot(nrm, dir);
+	const uint32_t mats = __float_as_uint(b.w);
+	const int mat_out = (dn < 0.0f) ? (int)(int16_t)(mats & 0xffffu) : (int)(int16_t)(mats >> 16);
+	if (mat_out == ignore_mat)
+		return false;
+	const float ee = a.w;
+	if (!(fabsf(dn) >= 1e-2f * ee))
+		return true;
+	const F3 sv = { cur.x - a.x, cur.y - a.y, cur.z - a.z };
+	const float slen = norm1(sv);
+	const float d0 = dot(sv, nrm);
+	const float m = 1e-2f * (1.0f + slen);
+	const float L = 1.01f * to_go + m;
+	const float fa = d0 - m * dn, fb = d0 + L * dn;
+	const float tol = 1e-3f * (slen + L) * ee;
+	return !((fa > tol && fb > tol) || (fa < -tol && fb < -tol));
+}
+
 // The neighbour across wall `target` (-1/-2/-4): flip that axis' bit at the
 // deepest level where the move stays inside the parent; location 1 (or 0)
 // means the ray left the octree.
@@ -680,15 +731,20 @@ __device__ __forceinline__ int wave_incl_sum(int v)
 // lists differ in length, and the wave runs as long as the longest.  Here
 // every round, each walking lane enters its next cell and counts its list
 // (g.cnt); the (ray, candidate) pairs of all lists are numbered by a prefix
-// sum over lanes and tested 64 at a time, one per lane, with the owning
-// ray's origin and direction fetched across lanes.  Each owner then takes,
-// in list order, the results that can hit (t > 0) and applies the
-// reference's running-minimum rule to them, so hit, distance and tie
-// choices are those of `propagate`.
+// sum over lanes and screened 64 at a time, one per lane, with the owning
+// ray's origin and direction fetched across lanes (cell_may_hit: material
+// test and a conservative plane test; about 2 % of the pairs pass on a
+// grating scan).  The pairs that pass are queued in LDS in pair order and
+// tested in full 64 at a time.  The results that can hit (t > 0) are then
+// taken in queue order, which is each owner's list order, and each owner
+// applies the reference's running-minimum rule to its own, so hit, distance
+// and tie choices are those of `propagate`.  (Testing every pair in full
+// took about 20 % more of the step kernel.)
 //
 // Every lane of the wave must call it (lanes with walk == false take part in
 // the shared tests only); a call from a partial wave falls back to
-// `propagate` lane by lane.  `owner_at` is 64 ints of LDS for this wave.
+// `propagate` lane by lane.  `owner_at` is 192 ints of LDS for this wave:
+// the owner table and the queue of pairs (triangle, owner lane).
 //
 // SHARED: every lane walks the same geometry.  Otherwise (lanes of different
 // scans) the candidate tests read the owning lane's triangle tables.
@@ -724,61 +780,143 @@ __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 
 		const int excl = incl - n;
 		off -= excl;  // pair p of this lane tests od[off + p]
 		const int total = __builtin_amdgcn_readlane(incl, 63);
-		for (int base = 0; base < total; base += 64) {
-			const int p = base + lane;
-			// Owner of pair p: the last lane whose pairs start at or before
-			// p.  Each lane whose pairs start inside the chunk marks that
-			// position in `owner_at`; position 0 holds the owner carried in
-			// from earlier (the last lane with excl <= base).
-			const uint64_t before = __ballot(excl <= base);
-			const int o0 = 63 - (int)__builtin_clzll(before);
-			owner_at[lane] = 0;
-			__builtin_amdgcn_wave_barrier();
-			const int e = excl - base;
-			if (n > 0 && e > 0 && e < 64)
-				owner_at[e] = lane + 1;
-			__builtin_amdgcn_wave_barrier();
-			const int mark = (lane == 0) ? o0 + 1 : owner_at[lane];
-			const uint64_t marks = __ballot(mark != 0);
-			const uint64_t upto_me = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-			const int at = 63 - (int)__builtin_clzll(marks & upto_me);
-			const int o = __shfl(mark, at) - 1;
-			const F3 oc = { __shfl(cur.x, o), __shfl(cur.y, o), __shfl(cur.z, o) };
-			const F3 odir = { __shfl(dir.x, o), __shfl(dir.y, o), __shfl(dir.z, o) };
-			const int oign_tri = __shfl(ignore_tri, o), oign_mat = __shfl(ignore_mat, o);
-			const int ooff = __shfl(off, o);
-			Geometry og = g;
-			if (!SHARED) {
-				og.od = (const int*)shfl_ptr(g.od, o);
-				og.tris = (const float4*)shfl_ptr(g.tris, o);
-			}
-			float t = -1.0f;
-			int ti = -1;
-			if (p < total) {
-				ti = og.od[ooff + p];
-				if (ti != oign_tri)
-					t = cell_candidate(og, ti, oc, odir, oign_mat);
-			}
-			// this lane's pairs within the chunk, lanes [lo, hi)
-			uint64_t todo = 0;
-			const uint64_t can_hit = __ballot(t > 0.0f);
-			if (active) {
-				const int lo = max(excl - base, 0), hi = min(incl - base, 64);
-				if (lo < hi) {
-					const uint64_t upto = (hi == 64) ? ~0ull : ((1ull << hi) - 1ull);
-					todo = can_hit & upto & ~((1ull << lo) - 1ull);
+		if constexpr (SHARED) {
+			int* q_ti = owner_at + 64;
+			int* q_own = owner_at + 128;
+			int qn = 0;   // queued pairs (wave-uniform)
+			// Second stage: test the queued pairs in full and fold the results
+			// that can hit, in queue order, into their owners' running minimum.
+			auto flush = [&]() {
+				__builtin_amdgcn_wave_barrier();
+				int ti = -1, own = 0;
+				if (lane < qn) {
+					ti = q_ti[lane];
+					own = q_own[lane];
 				}
-			}
-			while (__ballot(todo != 0ull) != 0ull) {
-				const int src = todo ? (int)__builtin_ctzll(todo) : lane;
-				const float ts = __shfl(t, src);
-				const int tis = __shfl(ti, src);
-				if (todo) {
-					if (ts <= isect + EPS) {
+				__builtin_amdgcn_wave_barrier();
+				const F3 oc = { __shfl(cur.x, own), __shfl(cur.y, own), __shfl(cur.z, own) };
+				const F3 odir = { __shfl(dir.x, own), __shfl(dir.y, own), __shfl(dir.z, own) };
+				float t = -1.0f;
+				if (lane < qn)
+					t = intersect_ray(load_tri(g.tris, ti), oc, odir);
+				uint64_t hits = __ballot(t > 0.0f);
+				while (hits) {
+					const int src = (int)__builtin_ctzll(hits);
+					hits &= hits - 1ull;
+					const float ts = __shfl(t, src);
+					const int tis = __shfl(ti, src), os = __shfl(own, src);
+					if (lane == os && ts <= isect + EPS) {
 						isect = ts;
 						target = tis;
 					}
-					todo &= todo - 1ull;
+				}
+			};
+			for (int base = 0; base < total; base += 64) {
+				const int p = base + lane;
+				// Owner of pair p: the last lane whose pairs start at or before
+				// p.  Each lane whose pairs start inside the chunk marks that
+				// position in `owner_at`; position 0 holds the owner carried in
+				// from earlier (the last lane with excl <= base).
+				const uint64_t before = __ballot(excl <= base);
+				const int o0 = 63 - (int)__builtin_clzll(before);
+				owner_at[lane] = 0;
+				__builtin_amdgcn_wave_barrier();
+				const int e = excl - base;
+				if (n > 0 && e > 0 && e < 64)
+					owner_at[e] = lane + 1;
+				__builtin_amdgcn_wave_barrier();
+				const int mark = (lane == 0) ? o0 + 1 : owner_at[lane];
+				const uint64_t marks = __ballot(mark != 0);
+				const uint64_t upto_me = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+				const int at = 63 - (int)__builtin_clzll(marks & upto_me);
+				const int o = __shfl(mark, at) - 1;
+				const F3 oc = { __shfl(cur.x, o), __shfl(cur.y, o), __shfl(cur.z, o) };
+				const F3 odir = { __shfl(dir.x, o), __shfl(dir.y, o), __shfl(dir.z, o) };
+				const int oign_tri = __shfl(ignore_tri, o), oign_mat = __shfl(ignore_mat, o);
+				const float oto = __shfl(to_go, o);
+				const int ooff = __shfl(off, o);
+				bool pass = false;
+				int ti = -1;
+				if (p < total) {
+					ti = g.od[ooff + p];
+					if (ti != oign_tri)
+						pass = cell_may_hit(g, ti, oc, odir, oign_mat, oto);
+				}
+				const uint64_t pm = __ballot(pass);
+				const int np = (int)__popcll(pm);
+				if (qn + np > 64) {
+					flush();
+					qn = 0;
+				}
+				if (pass) {
+					const int slot = qn + (int)__popcll(pm & ((1ull << lane) - 1ull));
+					q_ti[slot] = ti;
+					q_own[slot] = o;
+				}
+				qn += np;
+			}
+			if (qn > 0)
+				flush();
+		} else {
+			// Lanes of different worlds (the multi-world kernel, 4 waves per
+			// SIMD): every pair is tested in full; the screen's queue and
+			// second load measured slower there.
+			for (int base = 0; base < total; base += 64) {
+				const int p = base + lane;
+				// Owner of pair p: the last lane whose pairs start at or before
+				// p.  Each lane whose pairs start inside the chunk marks that
+				// position in `owner_at`; position 0 holds the owner carried in
+				// from earlier (the last lane with excl <= base).
+				const uint64_t before = __ballot(excl <= base);
+				const int o0 = 63 - (int)__builtin_clzll(before);
+				owner_at[lane] = 0;
+				__builtin_amdgcn_wave_barrier();
+				const int e = excl - base;
+				if (n > 0 && e > 0 && e < 64)
+					owner_at[e] = lane + 1;
+				__builtin_amdgcn_wave_barrier();
+				const int mark = (lane == 0) ? o0 + 1 : owner_at[lane];
+				const uint64_t marks = __ballot(mark != 0);
+				const uint64_t upto_me = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+				const int at = 63 - (int)__builtin_clzll(marks & upto_me);
+				const int o = __shfl(mark, at) - 1;
+				const F3 oc = { __shfl(cur.x, o), __shfl(cur.y, o), __shfl(cur.z, o) };
+				const F3 odir = { __shfl(dir.x, o), __shfl(dir.y, o), __shfl(dir.z, o) };
+				const int oign_tri = __shfl(ignore_tri, o), oign_mat = __shfl(ignore_mat, o);
+				const int ooff = __shfl(off, o);
+				Geometry og = g;
+				if (!SHARED) {
+					og.od = (const int*)shfl_ptr(g.od, o);
+					og.tris = (const float4*)shfl_ptr(g.tris, o);
+				}
+				float t = -1.0f;
+				int ti = -1;
+				if (p < total) {
+					ti = og.od[ooff + p];
+					if (ti != oign_tri)
+						t = cell_candidate(og, ti, oc, odir, oign_mat);
+				}
+				// this lane's pairs within the chunk, lanes [lo, hi)
+				uint64_t todo = 0;
+				const uint64_t can_hit = __ballot(t > 0.0f);
+				if (active) {
+					const int lo = max(excl - base, 0), hi = min(incl - base, 64);
+					if (lo < hi) {
+						const uint64_t upto = (hi == 64) ? ~0ull : ((1ull << hi) - 1ull);
+						todo = can_hit & upto & ~((1ull << lo) - 1ull);
+					}
+				}
+				while (__ballot(todo != 0ull) != 0ull) {
+					const int src = todo ? (int)__builtin_ctzll(todo) : lane;
+					const float ts = __shfl(t, src);
+					const int tis = __shfl(ti, src);
+					if (todo) {
+						if (ts <= isect + EPS) {
+							isect = ts;
+							target = tis;
+						}
+						todo &= todo - 1ull;
+					}
 				}
 			}
 		}
