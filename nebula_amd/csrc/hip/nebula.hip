@@ -988,7 +988,7 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 			stack.pop_back();
 			if (node + 8 > n_od)
 				return fail(NBL_ERR_FORMAT, "octree node out of range at " + std::to_string(node));
-			if (depth + 1 > PATH_LEVELS - 1)
+			if (depth + 1 > PATH_LEVELS)
 				return fail(NBL_ERR_FORMAT, "octree deeper than 21 levels");
 			for (int o = 0; o < 8; ++o) {
 				const int64_t v = od[node + o];

@@ -500,9 +500,9 @@ __device__ __forceinline__ bool in_domain(const Geometry& g, F3 p)
 		&& (p.z > c.z - h.z) && (p.z < c.z + h.z);
 }
 
-// Levels of the octree table cached per thread: path[l * stride] holds the
-// node index at level l (1..21) of the current cell.
-constexpr int PATH_LEVELS = 22;
+// Levels of the octree table cached per thread: path[(l - 1) * stride] holds
+// the node index at level l (1..21) of the current cell.
+constexpr int PATH_LEVELS = 21;
 
 // The pieces of the cell walk of geometry/octree.inl:32-192.
 //
@@ -539,9 +539,9 @@ __device__ __forceinline__ int cell_find(const Geometry& g, F3 cur, uint64_t& lo
 	int lev = (63 - clz) / 3;
 	int index = 0;
 	if (lev > 0) {
-		const int parent = (lev >= 2) ? path[(lev - 1) * stride] : 0;
+		const int parent = (lev >= 2) ? path[(lev - 2) * stride] : 0;
 		index = od[parent + (int)((unsigned)location & 7u)];
-		path[lev * stride] = index;
+		path[(lev - 1) * stride] = index;
 	}
 	while (index >= 0) {
 		unsigned oct = 0;
@@ -550,7 +550,8 @@ __device__ __forceinline__ int cell_find(const Geometry& g, F3 cur, uint64_t& lo
 		oct ^= (cur.z > az) ? 4u : 0u;
 		location = (location << 3) | oct;
 		index = od[index + (int)oct];
-		path[++lev * stride] = index;
+		path[lev * stride] = index;   // level lev + 1
+		++lev;
 		const float hx = hw * hs.x, hy = hw * hs.y, hz = hw * hs.z;
 		ax += (oct & 1u) ? hx : -hx;
 		ay += (oct & 2u) ? hy : -hy;
