@@ -1024,7 +1024,7 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 			return fail(NBL_ERR_ARG, "triangle " + std::to_string(i) + ": material id outside the 16-bit range");
 		}
 	if ((rc = dev_alloc(&g->od, (size_t)n_od)) || (rc = dev_alloc(&g->tris, (size_t)ntri * 3))
-		|| (rc = dev_alloc(&g->scr, (size_t)ntri * 2)) || (rc = dev_alloc(&g->cnt, (size_t)n_od))) {
+		|| (rc = dev_alloc(&g->scr, (size_t)n_od * 2)) || (rc = dev_alloc(&g->cnt, (size_t)n_od))) {
 		hipFree(g->od);
 		hipFree(g->tris);
 		hipFree(g->scr);
@@ -1036,7 +1036,7 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 	std::vector<int32_t> cnt((size_t)n_od, 0);
 	for (int64_t i = n_od - 1; i >= 0; --i)
 		cnt[i] = (od[i] < 0 || i + 1 == n_od) ? 0 : cnt[i + 1] + 1;
-	std::vector<float4> packed((size_t)ntri * 3), scr((size_t)ntri * 2);
+	std::vector<float4> packed((size_t)ntri * 3), tscr((size_t)ntri * 2);
 	for (int64_t i = 0; i < ntri; ++i) {
 		const nbl_triangle44& t = tris[i];
 		float ib, ob;
@@ -1055,9 +1055,18 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 		const uint32_t mats = ((uint32_t)t.material_in & 0xffffu) | ((uint32_t)t.material_out << 16);
 		float mf;
 		std::memcpy(&mf, &mats, 4);
-		scr[2 * i] = make_float4(t.r0[0], t.r0[1], t.r0[2], l1 * l2);
-		scr[2 * i + 1] = make_float4(nx, ny, nz, mf);
+		tscr[2 * i] = make_float4(t.r0[0], t.r0[1], t.r0[2], l1 * l2);
+		tscr[2 * i + 1] = make_float4(nx, ny, nz, mf);
 	}
+	// the screening records in table order: every entry that names a
+	// triangle (list entries; node offsets that happen to be below ntri get a
+	// record nobody reads)
+	std::vector<float4> scr((size_t)n_od * 2, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+	for (int64_t i = 0; i < n_od; ++i)
+		if (od[i] >= 0 && od[i] < ntri) {
+			scr[2 * i] = tscr[2 * (size_t)od[i]];
+			scr[2 * i + 1] = tscr[2 * (size_t)od[i] + 1];
+		}
 	HIP_OK(hipMemcpy(g->od, od, sizeof(int32_t) * n_od, hipMemcpyHostToDevice));
 	HIP_OK(hipMemcpy(g->tris, packed.data(), sizeof(float4) * packed.size(), hipMemcpyHostToDevice));
 	HIP_OK(hipMemcpy(g->scr, scr.data(), sizeof(float4) * scr.size(), hipMemcpyHostToDevice));

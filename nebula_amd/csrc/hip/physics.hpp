@@ -460,9 +460,12 @@ __device__ __forceinline__ Tri load_tri(const float4* t, int i)
 struct Geometry {
 	const int* od;
 	const float4* tris;
-	// Screening record per triangle, two float4: (r0, |e1|_1 |e2|_1) and
-	// (e1 x e2 as the triangle test computes it, material_in | material_out
-	// as two int16 in the bits of w).
+	// Screening record per entry of the octree table, two float4, filled at
+	// every position that names a triangle: (r0, |e1|_1 |e2|_1) and (e1 x e2
+	// as the triangle test computes it, material_in | material_out as two
+	// int16 in the bits of w).  Stored in list order, so the screen reads
+	// neighbouring records for neighbouring pairs, without waiting for the
+	// triangle index.
 	const float4* scr;
 	const int* cnt;  // cnt[i]: triangles listed from od[i] to the list's end
 	F3 center, halfsize;
@@ -594,7 +597,7 @@ __device__ __forceinline__ float cell_candidate(const Geometry& g, int ti, F3 cu
 }
 
 // First stage of the wave-wide walk's candidate test, on the 32-byte
-// screening record: the material test of cell_candidate (the same e1 x e2,
+// screening record of the list entry: the material test of cell_candidate (the same e1 x e2,
 // computed on the host with the same operations, so the same result) and a
 // plane test that drops
 // a pair only when its intersection distance cannot matter.  For the pair's
@@ -617,10 +620,10 @@ __device__ __forceinline__ float cell_candidate(const Geometry& g, int ti, F3 cu
 // candidate below to_go is taken in both folds and makes them equal again).
 // Pairs that pass are tested in full (intersect_ray) in the second stage.
 __device__ __forceinline__ float norm1(F3 a) { return fabsf(a.x) + fabsf(a.y) + fabsf(a.z); }
-__device__ __forceinline__ bool cell_may_hit(const Geometry& g, int ti, F3 cur, F3 dir, int ignore_mat,
+__device__ __forceinline__ bool cell_may_hit(const Geometry& g, int entry, F3 cur, F3 dir, int ignore_mat,
 	float to_go)
 {
-	const float4 a = g.scr[2 * ti], b = g.scr[2 * ti + 1];
+	const float4 a = g.scr[2 * entry], b = g.scr[2 * entry + 1];
 	const F3 nrm = { b.x, b.y, b.z };
 	const float dn = dot(nrm, dir);
 	const uint32_t mats = __float_as_uint(b.w);
@@ -840,8 +843,8 @@ __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 
 				int ti = -1;
 				if (p < total) {
 					ti = g.od[ooff + p];
-					if (ti != oign_tri)
-						pass = cell_may_hit(g, ti, oc, odir, oign_mat, oto);
+					const bool may = cell_may_hit(g, ooff + p, oc, odir, oign_mat, oto);
+					pass = may && ti != oign_tri;
 				}
 				const uint64_t pm = __ballot(pass);
 				const int np = (int)__popcll(pm);
