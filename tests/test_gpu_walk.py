@@ -140,3 +140,14 @@ def test_wave_walk_partial_waves(dev, n):
     it = np.full(n, -1, np.int32)
     im = np.full(n, -123, np.int32)
     _check(g, ow, pos, d, dist, it, im)
+
+
+def test_upload_rejects_material_ids_beyond_16_bits(dev):
+    """The walk's screening record holds both material ids as int16: an id
+    that does not fit is refused at upload (NBL_ERR_ARG), not truncated."""
+    tris = np.array(grating_tris(), copy=True)
+    tris["mat_in"][0] = 40000
+    with pytest.raises(A.NebulaError, match="16-bit"):
+        A.Geometry(dev, tris)
+    tris["mat_in"][0] = -123
+    A.Geometry(dev, tris)       # the special (negative) ids fit
