@@ -468,6 +468,9 @@ struct Geometry {
 	// triangle index.
 	const float4* scr;
 	const int* cnt;  // cnt[i]: triangles listed from od[i] to the list's end
+	// odc[i] = (od[i], cnt[-od[i]] when od[i] < 0 names a leaf list): the
+	// walk reads a child entry and its list length in one load.
+	const int2* odc;
 	F3 center, halfsize;
 	float max_extent;
 	int ntri;
@@ -516,11 +519,12 @@ constexpr int PATH_LEVELS = 21;
 // from the root in the reference's order, so every value is unchanged.
 //
 // cell_find: centre (ax, ay, az) and scale aw of the leaf holding `cur`,
-// extending `location` down to it; returns the start of its triangle list.
+// extending `location` down to it; returns the start of its triangle list
+// and its length in `count`.
 __device__ __forceinline__ int cell_find(const Geometry& g, F3 cur, uint64_t& location,
-	float& ax, float& ay, float& az, float& aw, int* __restrict__ path, int stride)
+	float& ax, float& ay, float& az, float& aw, int* __restrict__ path, int stride, int& count)
 {
-	const int* __restrict__ od = g.od;
+	const int2* __restrict__ odc = g.odc;
 	const F3 hs = g.halfsize;
 	// The reference adds aw * hs * (bit - 0.5) per level.  The product with
 	// +-0.5 is exact, and scaling by a power of two commutes with rounding
@@ -541,9 +545,12 @@ __device__ __forceinline__ int cell_find(const Geometry& g, F3 cur, uint64_t& lo
 	// levels 1..L-1 are unchanged ancestors; level L holds the new octant
 	int lev = (63 - clz) / 3;
 	int index = 0;
+	count = 0;
 	if (lev > 0) {
 		const int parent = (lev >= 2) ? path[(lev - 2) * stride] : 0;
-		index = od[parent + (int)((unsigned)location & 7u)];
+		const int2 e = odc[parent + (int)((unsigned)location & 7u)];
+		index = e.x;
+		count = e.y;
 		path[(lev - 1) * stride] = index;
 	}
 	while (index >= 0) {
@@ -552,7 +559,9 @@ __device__ __forceinline__ int cell_find(const Geometry& g, F3 cur, uint64_t& lo
 		oct ^= (cur.y > ay) ? 2u : 0u;
 		oct ^= (cur.z > az) ? 4u : 0u;
 		location = (location << 3) | oct;
-		index = od[index + (int)oct];
+		const int2 e = odc[index + (int)oct];
+		index = e.x;
+		count = e.y;
 		path[lev * stride] = index;   // level lev + 1
 		++lev;
 		const float hx = hw * hs.x, hy = hw * hs.y, hz = hw * hs.z;
@@ -674,7 +683,8 @@ __device__ inline int propagate(const Geometry& g, F3 start, F3 dir, float dista
 	uint32_t cells = 0;
 	do {
 		float ax, ay, az, aw;
-		int index = cell_find(g, cur, location, ax, ay, az, aw, path, stride);
+		int count;
+		int index = cell_find(g, cur, location, ax, ay, az, aw, path, stride, count);
 		++cells;
 		float isect;
 		int target;
@@ -775,10 +785,9 @@ __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 
 		float isect = 0.0f;
 		if (active) {
 			float ax, ay, az, aw;
-			off = cell_find(g, cur, location, ax, ay, az, aw, path, stride);
+			off = cell_find(g, cur, location, ax, ay, az, aw, path, stride, n);
 			++cells;
 			cell_exit(g, cur, dir, ax, ay, az, aw, isect, target);
-			n = g.cnt[off];
 		}
 		const int incl = wave_incl_sum(n);
 		const int excl = incl - n;
