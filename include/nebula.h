@@ -196,8 +196,8 @@ int nbl_device_synchronize(nbl_device* dev);
 
 /* octree<true>::create (geometry/octree.inl:282-303): upload a linearised
  * octree (layout of octree_builder::linearized_octree).  Device memory:
- * 4 B (table) + 4 B (list lengths) + 8 B (entries with list lengths) +
- * 32 B (screening records) per table entry and 48 B per triangle.  Material ids must fit in 16 bits
+ * 4 B (table) + 8 B (entries with their leaf-list lengths) + 32 B
+ * (screening records) per table entry and 48 B per triangle.  Material ids must fit in 16 bits
  * (NBL_ERR_ARG otherwise). */
 int nbl_geometry_upload(nbl_device* dev, const int32_t* octree_data, int64_t n_octree,
 	const nbl_triangle44* tris, int64_t ntri, const float center[3], const float halfsize[3],

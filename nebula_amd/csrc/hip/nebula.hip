@@ -765,7 +765,6 @@ struct nbl_geometry {
 	int* od;
 	float4* tris;
 	float4* scr;
-	int* cnt;
 	int2* odc;
 	Geometry g;
 	int max_material;   // largest physical material index named by a triangle
@@ -1025,12 +1024,10 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 			return fail(NBL_ERR_ARG, "triangle " + std::to_string(i) + ": material id outside the 16-bit range");
 		}
 	if ((rc = dev_alloc(&g->od, (size_t)n_od)) || (rc = dev_alloc(&g->tris, (size_t)ntri * 3))
-		|| (rc = dev_alloc(&g->scr, (size_t)n_od * 2)) || (rc = dev_alloc(&g->cnt, (size_t)n_od))
-		|| (rc = dev_alloc(&g->odc, (size_t)n_od))) {
+		|| (rc = dev_alloc(&g->scr, (size_t)n_od * 2)) || (rc = dev_alloc(&g->odc, (size_t)n_od))) {
 		hipFree(g->od);
 		hipFree(g->tris);
 		hipFree(g->scr);
-		hipFree(g->cnt);
 		delete g;
 		return rc;
 	}
@@ -1073,7 +1070,6 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 	HIP_OK(hipMemcpy(g->od, od, sizeof(int32_t) * n_od, hipMemcpyHostToDevice));
 	HIP_OK(hipMemcpy(g->tris, packed.data(), sizeof(float4) * packed.size(), hipMemcpyHostToDevice));
 	HIP_OK(hipMemcpy(g->scr, scr.data(), sizeof(float4) * scr.size(), hipMemcpyHostToDevice));
-	HIP_OK(hipMemcpy(g->cnt, cnt.data(), sizeof(int32_t) * n_od, hipMemcpyHostToDevice));
 	// child entries with their list lengths (leaf offsets are checked above
 	// to lie in (0, n_od))
 	std::vector<int2> odc((size_t)n_od);
@@ -1085,7 +1081,6 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 	g->g.od = g->od;
 	g->g.tris = g->tris;
 	g->g.scr = g->scr;
-	g->g.cnt = g->cnt;
 	g->g.odc = g->odc;
 	g->g.center = { center[0], center[1], center[2] };
 	g->g.halfsize = { halfsize[0], halfsize[1], halfsize[2] };
@@ -1109,7 +1104,6 @@ int nbl_geometry_destroy(nbl_geometry* g)
 	hipFree(g->od);
 	hipFree(g->tris);
 	hipFree(g->scr);
-	hipFree(g->cnt);
 	hipFree(g->odc);
 	delete g;
 	return NBL_OK;

@@ -467,9 +467,9 @@ struct Geometry {
 	// neighbouring records for neighbouring pairs, without waiting for the
 	// triangle index.
 	const float4* scr;
-	const int* cnt;  // cnt[i]: triangles listed from od[i] to the list's end
-	// odc[i] = (od[i], cnt[-od[i]] when od[i] < 0 names a leaf list): the
-	// walk reads a child entry and its list length in one load.
+	// odc[i] = (od[i], length of the leaf list at -od[i] when od[i] < 0
+	// names one): the walk reads a child entry and its list length in one
+	// load.
 	const int2* odc;
 	F3 center, halfsize;
 	float max_extent;
@@ -744,7 +744,7 @@ __device__ __forceinline__ int wave_incl_sum(int v)
 // One lane walking its own cell's triangle list leaves most of the wave idle:
 // lists differ in length, and the wave runs as long as the longest.  Here
 // every round, each walking lane enters its next cell and counts its list
-// (g.cnt); the (ray, candidate) pairs of all lists are numbered by a prefix
+// (g.odc); the (ray, candidate) pairs of all lists are numbered by a prefix
 // sum over lanes and screened 64 at a time, one per lane, with the owning
 // ray's origin and direction fetched across lanes (cell_may_hit: material
 // test and a conservative plane test; about 2 % of the pairs pass on a
