@@ -738,6 +738,28 @@ __device__ __forceinline__ int wave_incl_sum(int v)
 	return v;
 }
 
+// Owner of pair base + lane in a round of the wave-wide walk: the last lane
+// whose pairs start at or before it (excl = first pair of this lane's list,
+// n = its length).  Each lane whose pairs start inside the chunk marks that
+// position in `owner_at` (64 ints of LDS); position 0 holds the owner carried
+// in from earlier (the last lane with excl <= base).
+__device__ __forceinline__ int chunk_owner(int base, int excl, int n, int* owner_at, int lane)
+{
+	const uint64_t before = __ballot(excl <= base);
+	const int o0 = 63 - (int)__builtin_clzll(before);
+	owner_at[lane] = 0;
+	__builtin_amdgcn_wave_barrier();
+	const int e = excl - base;
+	if (n > 0 && e > 0 && e < 64)
+		owner_at[e] = lane + 1;
+	__builtin_amdgcn_wave_barrier();
+	const int mark = (lane == 0) ? o0 + 1 : owner_at[lane];
+	const uint64_t marks = __ballot(mark != 0);
+	const uint64_t upto_me = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+	const int at = 63 - (int)__builtin_clzll(marks & upto_me);
+	return __shfl(mark, at) - 1;
+}
+
 // The same walk for the rays of a whole wave, with the candidate tests shared
 // out over all 64 lanes.
 //
@@ -826,23 +848,7 @@ __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 
 			};
 			for (int base = 0; base < total; base += 64) {
 				const int p = base + lane;
-				// Owner of pair p: the last lane whose pairs start at or before
-				// p.  Each lane whose pairs start inside the chunk marks that
-				// position in `owner_at`; position 0 holds the owner carried in
-				// from earlier (the last lane with excl <= base).
-				const uint64_t before = __ballot(excl <= base);
-				const int o0 = 63 - (int)__builtin_clzll(before);
-				owner_at[lane] = 0;
-				__builtin_amdgcn_wave_barrier();
-				const int e = excl - base;
-				if (n > 0 && e > 0 && e < 64)
-					owner_at[e] = lane + 1;
-				__builtin_amdgcn_wave_barrier();
-				const int mark = (lane == 0) ? o0 + 1 : owner_at[lane];
-				const uint64_t marks = __ballot(mark != 0);
-				const uint64_t upto_me = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-				const int at = 63 - (int)__builtin_clzll(marks & upto_me);
-				const int o = __shfl(mark, at) - 1;
+				const int o = chunk_owner(base, excl, n, owner_at, lane);
 				const F3 oc = { __shfl(cur.x, o), __shfl(cur.y, o), __shfl(cur.z, o) };
 				const F3 odir = { __shfl(dir.x, o), __shfl(dir.y, o), __shfl(dir.z, o) };
 				const int oign_tri = __shfl(ignore_tri, o), oign_mat = __shfl(ignore_mat, o);
@@ -876,32 +882,14 @@ __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 
 			// second load measured slower there.
 			for (int base = 0; base < total; base += 64) {
 				const int p = base + lane;
-				// Owner of pair p: the last lane whose pairs start at or before
-				// p.  Each lane whose pairs start inside the chunk marks that
-				// position in `owner_at`; position 0 holds the owner carried in
-				// from earlier (the last lane with excl <= base).
-				const uint64_t before = __ballot(excl <= base);
-				const int o0 = 63 - (int)__builtin_clzll(before);
-				owner_at[lane] = 0;
-				__builtin_amdgcn_wave_barrier();
-				const int e = excl - base;
-				if (n > 0 && e > 0 && e < 64)
-					owner_at[e] = lane + 1;
-				__builtin_amdgcn_wave_barrier();
-				const int mark = (lane == 0) ? o0 + 1 : owner_at[lane];
-				const uint64_t marks = __ballot(mark != 0);
-				const uint64_t upto_me = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-				const int at = 63 - (int)__builtin_clzll(marks & upto_me);
-				const int o = __shfl(mark, at) - 1;
+				const int o = chunk_owner(base, excl, n, owner_at, lane);
 				const F3 oc = { __shfl(cur.x, o), __shfl(cur.y, o), __shfl(cur.z, o) };
 				const F3 odir = { __shfl(dir.x, o), __shfl(dir.y, o), __shfl(dir.z, o) };
 				const int oign_tri = __shfl(ignore_tri, o), oign_mat = __shfl(ignore_mat, o);
 				const int ooff = __shfl(off, o);
 				Geometry og = g;
-				if (!SHARED) {
-					og.od = (const int*)shfl_ptr(g.od, o);
-					og.tris = (const float4*)shfl_ptr(g.tris, o);
-				}
+				og.od = (const int*)shfl_ptr(g.od, o);
+				og.tris = (const float4*)shfl_ptr(g.tris, o);
 				float t = -1.0f;
 				int ti = -1;
 				if (p < total) {
