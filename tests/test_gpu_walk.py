@@ -6,6 +6,8 @@ chosen to reach the walk's corner cases:
   report hits within EPS of each other and the reference's running-minimum
   rule (t <= isect + EPS replaces the current hit) decides by list order;
 - rays grazing a triangle's plane (det near EPS);
+- free paths that end just short of, on or just past a triangle, where
+  the walk's plane screen must keep the pair;
 - axis-parallel rays (zero direction components: infinite wall distances);
 - long flights through vacuum that cross many cells and leave the octree;
 - query counts that leave a partial last wave, down to a single ray.
@@ -74,6 +76,24 @@ def _grazing_rays(tris, n, rng):
     return pos, d, (back * 2.0).astype(np.float32)
 
 
+def _step_end_rays(tris, n, rng):
+    """Rays toward a triangle's interior whose free path ends just short of
+    it, on it or just past it (relative offsets down to 1e-7): the walk's
+    screen must keep every pair whose hit decides between "hit" and "no hit
+    within the step"."""
+    k = rng.integers(0, len(tris), n)
+    r0 = tris["r0"][k].astype(np.float64)
+    e1 = tris["e1"][k].astype(np.float64)
+    e2 = tris["e2"][k].astype(np.float64)
+    a, b = rng.random(n) * 0.5, rng.random(n) * 0.5
+    target = r0 + a[:, None] * e1 + b[:, None] * e2
+    d = _unit(rng.standard_normal((n, 3)))
+    back = rng.uniform(0.05, 20.0, n)
+    pos = (target - d * back[:, None]).astype(np.float32)
+    rel = rng.choice([-1e-3, -1e-5, -1e-7, 0.0, 1e-7, 1e-5, 1e-3], n)
+    return pos, d, (back * (1.0 + rel)).astype(np.float32)
+
+
 def _axis_rays(center, halfsize, n, rng):
     pos = (center + (rng.random((n, 3)) * 2 - 1) * halfsize * 0.9).astype(np.float32)
     axes = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
@@ -112,7 +132,7 @@ def test_wave_walk_corner_cases_bit_exact(dev, name):
     g = A.Geometry(dev, tris)
     ow = O.OracleWorld(tris, [material("si"), material("pmma"), material("si")])
     rng = np.random.default_rng(17)
-    parts = [_edge_rays(tris, 6000, rng), _grazing_rays(tris, 3000, rng),
+    parts = [_edge_rays(tris, 6000, rng), _grazing_rays(tris, 3000, rng), _step_end_rays(tris, 4000, rng),
              _axis_rays(g.center, g.halfsize, 2000, rng), _vacuum_flights(g.center, g.halfsize, 2000, rng),
              random_rays(g.center, g.halfsize, 3000, seed=23)]
     pos = np.concatenate([p[0] for p in parts])
