@@ -23,8 +23,9 @@ The two other parts of the metric are reported beside it:
   batched - 64 independent scans of uneven size (seeded random gratings and
             contact-hole arrays, 1e3-2e5 triangles, 200 eV-20 keV, 32^2-256^2
             pixels at 10-200 primaries per pixel) tracked together by one
-            driver (one world per scan, graph-replayed iterations, autotuned
-            launch) against the same scans run one after another on the
+            driver (one world per scan, graph-replayed iterations, steps per
+            launch followed online) against the same scans run one after
+            another on the
             plain single-scan path; per-scan yields must be identical.
   grad    - the grating at 64x64 pixels with ~1e6 primaries: forward run vs
             the likelihood-ratio replay that yields d(per-pixel SE
@@ -137,8 +138,8 @@ def cpu_baseline(mesh, pri_full, mats, target_s=20.0, threads=None):
 
 
 def bench_batched(dev, scale=1.0):
-    """64 uneven scans: batched (one driver, worlds, graphs, autotune) vs the
-    plain single-scan path run scan by scan."""
+    """64 uneven scans: batched (one driver, worlds, graphs, steps per launch
+    followed online) vs the plain single-scan path run scan by scan."""
     from nebula_amd import batch as B
     t0 = time.perf_counter()
     scans = W.batch_scans(64, seed=1, per_pixel_scale=scale)
