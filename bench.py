@@ -25,8 +25,8 @@ The two other parts of the metric are reported beside it:
             pixels at 10-200 primaries per pixel) tracked together by one
             driver (one world per scan, graph-replayed iterations, steps per
             launch followed online) against the same scans run one after
-            another on the
-            plain single-scan path; per-scan yields must be identical.
+            another on the plain single-scan path; per-scan yields must be
+            identical.
   grad    - the grating at 64x64 pixels with ~1e6 primaries: forward run vs
             the likelihood-ratio replay that yields d(per-pixel SE
             yield)/d(theta) for 5 parameters (Si and PMMA elastic/inelastic
