@@ -301,6 +301,40 @@ def bench_grad_reference(dev, replicas=100, fd_h=1e-3):
             "per_pixel_rel_l2": float(np.linalg.norm(j0 - j_ref) / np.linalg.norm(j_ref))}
 
 
+def bench_cli(capacity, steps_per_launch, pixels=256, per_pixel=100):
+    """The command-line drop-in (nebula_amd/lib/nebula_gpu, the reference's
+    nebula_gpu interface) on the bench scan: .tri/.mat/.pri files written
+    first, records to stdout (read into a pipe and counted).  Its own timing
+    lines give the prescan and the frame loop ("Simulation"), the host
+    phases (file loads, octree build) are reported beside them."""
+    from nebula_amd import _lib
+    cli = os.path.join(_lib.LIB_DIR, "nebula_gpu")
+    mesh, pri, mats = workload(pixels, pixels, per_pixel)
+    tmp = tempfile.mkdtemp()
+    W.write_tri(os.path.join(tmp, "g.tri"), mesh)
+    W.write_pri(os.path.join(tmp, "s.pri"), pri)
+    matfiles = []
+    for i, m in enumerate(mats):
+        matfiles.append(os.path.join(tmp, f"m{i}.mat"))
+        A.write_mat_file(matfiles[-1], m)
+    print(f"cli: {len(pri)} primaries written", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    r = subprocess.run([cli, "--capacity", str(capacity), "--steps-per-launch", str(steps_per_launch),
+                        os.path.join(tmp, "g.tri"), os.path.join(tmp, "s.pri"), *matfiles],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=600)
+    wall = time.perf_counter() - t0
+    err = r.stderr.decode()
+    if r.returncode != 0:
+        raise RuntimeError(err[-2000:])
+    phases = {m.group(1).strip(): float(m.group(2)) for m in re.finditer(r"^\s*([A-Za-z ]+): ([0-9.]+)$", err, re.M)}
+    n_rec = len(r.stdout) // 36
+    sim = phases.get("Simulation", float("nan")) + phases.get("Prescan", 0.0)
+    return {"workload": f"nebula_gpu on the grating scan, {pixels}x{pixels} px x {per_pixel} primaries at 300 eV",
+            "primaries": int(len(pri)), "records": int(n_rec), "capacity": capacity,
+            "steps_per_launch": steps_per_launch, "phases_s": phases, "process_wall_s": wall,
+            "primaries_per_s_prescan_plus_simulation": len(pri) / sim}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -327,6 +361,8 @@ def main():
     ap.add_argument("--grad-reference", type=int, default=0,
                     help="run only the gradient check against a reference gradient from this many "
                          "times the primaries (fresh replicas of the 1e6-primary scan); print its JSON line")
+    ap.add_argument("--cli", action="store_true",
+                    help="run only the command-line drop-in (nebula_gpu) on the scan and print its JSON line")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "step_pair_traffic.json"),
                     help="measured HBM bytes per k_step + k_inel launch pair (from rocprofv3 --pmc runs)")
     args = ap.parse_args()
@@ -340,6 +376,13 @@ def main():
                           "value": r["primaries_per_s"], "unit": "primaries/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "f32", "data": "synthetic", "large": r}),
               flush=True)
+        return
+    if args.cli:
+        r = bench_cli(args.capacity, args.steps_per_launch, args.pixels, args.per_pixel)
+        print(json.dumps({"metric": "nebula_gpu command line: primaries/s (prescan + simulation phases)",
+                          "value": r["primaries_per_s_prescan_plus_simulation"], "unit": "primaries/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+                          "cli": r}), flush=True)
         return
     if args.grad_reference > 0:
         r = bench_grad_reference(A.Device(0), args.grad_reference)
