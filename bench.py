@@ -328,6 +328,9 @@ def bench_cli(capacity, steps_per_launch, pixels=256, per_pixel=100):
         raise RuntimeError(err[-2000:])
     phases = {m.group(1).strip(): float(m.group(2)) for m in re.finditer(r"^\s*([A-Za-z ]+): ([0-9.]+)$", err, re.M)}
     n_rec = len(r.stdout) // 36
+    fm = re.search(r"frame_size = (\d+) \| batch_size = (\d+)", err)
+    if fm:
+        phases["frame_size"], phases["batch_size"] = int(fm.group(1)), int(fm.group(2))
     sim = phases.get("Simulation", float("nan")) + phases.get("Prescan", 0.0)
     return {"workload": f"nebula_gpu on the grating scan, {pixels}x{pixels} px x {per_pixel} primaries at 300 eV",
             "primaries": int(len(pri)), "records": int(n_rec), "capacity": capacity,
