@@ -40,6 +40,7 @@ import argparse
 import json
 import os
 import re
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -72,9 +73,16 @@ def workload(nx=256, ny=256, per_pixel=100, energy=300.0, seed=1):
 
 def cpu_baseline(mesh, pri_full, mats, target_s=20.0, threads=None):
     """Time the reference CPU binary (or the oracle) on a crop of the scan."""
+    tmp = tempfile.mkdtemp()
+    try:
+        return _cpu_baseline(mesh, pri_full, mats, target_s, threads, tmp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def _cpu_baseline(mesh, pri_full, mats, target_s, threads, tmp):
     threads = threads or min(16, os.cpu_count() or 1)
     ref_bin = os.path.join(ROOT, "oracle", "_ref", "nebula_cpu_mt")
-    tmp = tempfile.mkdtemp()
     # crop: a block of pixels from the scan's centre, all primaries of each
     ix, iy = pri_full["px"], pri_full["py"]
 
@@ -307,10 +315,17 @@ def bench_cli(capacity, steps_per_launch, pixels=256, per_pixel=100):
     first, records to stdout (read into a pipe and counted).  Its own timing
     lines give the prescan and the frame loop ("Simulation"), the host
     phases (file loads, octree build) are reported beside them."""
+    tmp = tempfile.mkdtemp()
+    try:
+        return _bench_cli(capacity, steps_per_launch, pixels, per_pixel, tmp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+def _bench_cli(capacity, steps_per_launch, pixels, per_pixel, tmp):
     from nebula_amd import _lib
     cli = os.path.join(_lib.LIB_DIR, "nebula_gpu")
     mesh, pri, mats = workload(pixels, pixels, per_pixel)
-    tmp = tempfile.mkdtemp()
     W.write_tri(os.path.join(tmp, "g.tri"), mesh)
     W.write_pri(os.path.join(tmp, "s.pri"), pri)
     matfiles = []
