@@ -100,6 +100,42 @@ def slab_geometry(size_nm: float = 1000.0, detector_z: float = 10.0) -> Mesh:
     return mb.mesh()
 
 
+def special_geometry(half: float = 40.0, depth: float = 60.0) -> Mesh:
+    """A block of material 0 (x, y in [-half, half], z in [-depth, 0]) under
+    planes of every special outgoing material of core/material_manager.h:
+    77-86, so that tracking meets each branch of boundary_intersect.h:77-100:
+
+    - NOP planes inside the block (z = -4) and above it (z = +3, x > 0):
+      the electron stops on them and goes on unchanged;
+    - a DETECTOR_LT50 plane at z = +6 (detects electrons below 50 eV, lets
+      the others through);
+    - at z = +12 a DETECTOR_GE50 plane over x < 0 (detects 50 eV and above)
+      and a MIRROR over x > 0 (reflects everything back down);
+    - TERMINATOR walls at x = +-half and y = +-half between z = 0 and +20;
+    - a DETECTOR plane at z = +20 over everything.
+
+    Every plane has the same special material on both sides."""
+    mb = _MeshBuilder()
+    h = half
+    _box(mb, (-h, -h, -depth), (h, h, 0.0), 0, VACUUM)
+
+    def plane(x0, x1, y0, y1, z, m):
+        mb.quad((x0, y0, z), (x1, y0, z), (x1, y1, z), (x0, y1, z), m, m)
+
+    plane(-h, h, -h, h, -4.0, NOP)
+    plane(0.0, h, -h, h, 3.0, NOP)
+    plane(-h, h, -h, h, 6.0, DETECTOR_LT50)
+    plane(-h, 0.0, -h, h, 12.0, DETECTOR_GE50)
+    plane(0.0, h, -h, h, 12.0, MIRROR)
+    plane(-h, h, -h, h, 20.0, DETECTOR)
+    T = TERMINATOR
+    mb.quad((-h, -h, 0.0), (-h, h, 0.0), (-h, h, 20.0), (-h, -h, 20.0), T, T)
+    mb.quad((h, -h, 0.0), (h, -h, 20.0), (h, h, 20.0), (h, h, 0.0), T, T)
+    mb.quad((-h, -h, 0.0), (-h, -h, 20.0), (h, -h, 20.0), (h, -h, 0.0), T, T)
+    mb.quad((-h, h, 0.0), (h, h, 0.0), (h, h, 20.0), (-h, h, 20.0), T, T)
+    return mb.mesh()
+
+
 def gaussian_random_field(shape, spacing, sigma, corr_len, rng):
     """Zero-mean Gaussian field with standard deviation `sigma` and Gaussian
     autocorrelation exp(-r^2/corr_len^2), by filtering white noise in Fourier

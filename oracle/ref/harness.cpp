@@ -14,6 +14,11 @@
 //             energydeposit_cpu_driver over all primaries, one thread:
 //             deposit records (float pos[3], E, dE, uint32 tag) in callback
 //             order, int64 count; then detected records as for simulate
+//   stepx     <tris.bin> <variant> <threshold> <seed> <states.bin> <out.bin> <mat>...
+//   simulatex <tris.bin> <variant> <threshold> <seed> <pri.bin> <out.bin> <mat>...
+//             step / simulate with the physics of a named variant (the
+//             template flags of physics_config.h set otherwise, see
+//             `variant` below) and an energy threshold (eV)
 //   pri       <in.pri> <prescan_size> <sort 0|1> <out.bin>
 //             load_pri_file, optional sort_pri_file, prescan_shuffle, in the
 //             order of gpu_main.cu:364-367; out = the 36-byte records
@@ -45,6 +50,25 @@ using penn_list = scatter_list<nbl::scatter::full_penn<false, true>, nbl::scatte
 using kieft_list = scatter_list<nbl::scatter::kieft_inelastic<false, true, true, true, true>,
 	nbl::scatter::kieft_elastic<false, true, true>>;
 using geometry_t = nbl::geometry::octree<false>;
+
+// Non-default physics of the stepx / simulatex commands.  Each names the
+// template flags it sets (physics/full_penn.h:18-20, kieft/inelastic.h:
+// 18-24, kieft/elastic.h:20-22, boundary_intersect.h:10-14):
+//   penn_abs   the default Penn list with interface absorption on
+//              (boundary_intersect<true, true, true>);
+//   penn_alt   full_penn<no secondaries>, kieft_elastic<phonon loss, no
+//              recoil>, boundary_intersect<quantum, no refraction, absorption>;
+//   kieft_alt  kieft_inelastic<no optical phonon, secondaries, no
+//              instantaneous momentum, no momentum conservation>,
+//              kieft_elastic<no phonon loss, no recoil>,
+//              boundary_intersect<classical, no refraction, no absorption>.
+using intersect_abs = boundary_intersect<true, true, true>;
+using penn_alt_list = scatter_list<nbl::scatter::full_penn<false, false>,
+	nbl::scatter::kieft_elastic<false, true, false>>;
+using intersect_penn_alt = boundary_intersect<true, false, true>;
+using kieft_alt_list = scatter_list<nbl::scatter::kieft_inelastic<false, false, true, false, false>,
+	nbl::scatter::kieft_elastic<false, false, false>>;
+using intersect_kieft_alt = boundary_intersect<false, false, false>;
 
 template <typename T>
 static std::vector<T> read_vec(const char* path)
@@ -224,7 +248,7 @@ template <typename L> static int cmd_probe(char** a)
 // then for a secondary: float pos[3], dir[3], E, int32 material.
 struct St { float pos[3], dir[3], E; int32_t mat, last_tri; };
 
-template <typename L> static int cmd_step(char** a, int nmat)
+template <typename L, typename I = intersect_t> static int cmd_step(char** a, int nmat, float threshold = 0)
 {
 	auto tris = read_tris(a[0]);
 	const uint32_t seed = (uint32_t)std::stoul(a[2]);
@@ -233,11 +257,11 @@ template <typename L> static int cmd_step(char** a, int nmat)
 	for (int i = 0; i < nmat; ++i)
 		mats.push_back(material<L>(nbl::hdf5_file(a[5 + i])));
 	geometry_t g = geometry_t::create(tris);
-	intersect_t inter;
+	I inter;
 	Out o(a[4]);
 	for (size_t i = 0; i < states.size(); ++i) {
 		const St& s = states[i];
-		nbl::drivers::simple_cpu_driver<L, intersect_t, geometry_t> d(g, inter, mats, 0, seed + (uint32_t)i);
+		nbl::drivers::simple_cpu_driver<L, I, geometry_t> d(g, inter, mats, threshold, seed + (uint32_t)i);
 		particle p{ { s.pos[0], s.pos[1], s.pos[2] }, { s.dir[0], s.dir[1], s.dir[2] }, s.E };
 		uint32_t tag = 0;
 		d.push(&p, &tag, 1);
@@ -265,7 +289,7 @@ template <typename L> static int cmd_step(char** a, int nmat)
 
 // pri.bin: .pri records.  Output: detected records (float pos[3], dir[3], E,
 // uint32 tag) in the driver's flush order; then a trailer int64 count.
-template <typename L> static int cmd_simulate(char** a, int nmat)
+template <typename L, typename I = intersect_t> static int cmd_simulate(char** a, int nmat, float threshold = 0)
 {
 	auto tris = read_tris(a[0]);
 	const uint32_t seed = (uint32_t)std::stoul(a[2]);
@@ -276,8 +300,8 @@ template <typename L> static int cmd_simulate(char** a, int nmat)
 	std::vector<particle> prim;
 	std::vector<int2> pix;
 	std::tie(prim, pix) = nbl::load_pri_file(a[3], g.AABB_min(), g.AABB_max(), 1e9f);
-	intersect_t inter;
-	nbl::drivers::simple_cpu_driver<L, intersect_t, geometry_t> d(g, inter, mats, 0, seed);
+	I inter;
+	nbl::drivers::simple_cpu_driver<L, I, geometry_t> d(g, inter, mats, threshold, seed);
 	Out o(a[4]);
 	int64_t n = 0;
 	for (size_t i = 0; i < prim.size(); ++i) {
@@ -331,6 +355,38 @@ template <typename L> static int cmd_edep(char** a, int nmat)
 	return 0;
 }
 
+// stepx / simulatex: argv a = <tris> <variant> <threshold> <seed> ...; the
+// variant and threshold are taken out so that the rest reads as for step /
+// simulate.
+static int cmd_variant(bool sim, char** a, int na)
+{
+	const std::string v = a[1];
+	const float threshold = std::stof(a[2]);
+	char* b[64];
+	const int nb = na - 1;
+	if (nb > 64)
+		throw std::runtime_error("too many arguments");
+	b[0] = a[0];
+	b[1] = a[1];
+	for (int i = 2; i < nb; ++i)
+		b[i] = a[i + 1];
+	const int nmat = nb - 5;
+	if (v == "penn")
+		return sim ? cmd_simulate<penn_list>(b, nmat, threshold) : cmd_step<penn_list>(b, nmat, threshold);
+	if (v == "kieft")
+		return sim ? cmd_simulate<kieft_list>(b, nmat, threshold) : cmd_step<kieft_list>(b, nmat, threshold);
+	if (v == "penn_abs")
+		return sim ? cmd_simulate<penn_list, intersect_abs>(b, nmat, threshold)
+			: cmd_step<penn_list, intersect_abs>(b, nmat, threshold);
+	if (v == "penn_alt")
+		return sim ? cmd_simulate<penn_alt_list, intersect_penn_alt>(b, nmat, threshold)
+			: cmd_step<penn_alt_list, intersect_penn_alt>(b, nmat, threshold);
+	if (v == "kieft_alt")
+		return sim ? cmd_simulate<kieft_alt_list, intersect_kieft_alt>(b, nmat, threshold)
+			: cmd_step<kieft_alt_list, intersect_kieft_alt>(b, nmat, threshold);
+	throw std::runtime_error("unknown variant " + v);
+}
+
 static int cmd_pri(char** a)
 {
 	const float big = std::numeric_limits<float>::max();
@@ -362,6 +418,8 @@ int main(int argc, char** argv)
 		if (c == "propagate") return cmd_propagate(a);
 		if (c == "intersect") return cmd_intersect(a);
 		if (c == "pri") return cmd_pri(a);
+		if (c == "stepx") return cmd_variant(false, a, na);
+		if (c == "simulatex") return cmd_variant(true, a, na);
 		const bool penn = na > 1 && std::string(a[1]) == "penn";
 		if (c == "material") return penn ? cmd_material<penn_list>(a) : cmd_material<kieft_list>(a);
 		if (c == "probe") return penn ? cmd_probe<penn_list>(a) : cmd_probe<kieft_list>(a);

@@ -9,6 +9,13 @@ files by nebula_amd's HDF5 writer, seeded rays / states / beams), passed to
 the harness, and stored together with the outputs as compressed .npz
 fixtures.  Large prepared tables are stored as SHA-256 digests plus a sampled
 subset.
+
+    python tests/golden/make_golden.py variants
+
+writes reference_variants.npz only: single steps and whole cascades with the
+physics variants of the harness (non-default template flags, interface
+absorption, an energy threshold) on special_geometry, which holds every
+special outgoing material, and on the small grating.
 """
 from __future__ import annotations
 
@@ -82,6 +89,111 @@ def states_for(n, seed, zlo=-55.0, zhi=-0.01):
     s["mat"] = 0
     s["last_tri"] = -1
     return s
+
+
+def parse_steps(path, n):
+    """stepx / step output: per state a header, then its secondaries."""
+    raw = np.fromfile(path, np.uint8)
+    off = 0
+    head = np.dtype([("status", "<i4"), ("next", "<i4"), ("mat", "<i4"), ("last_tri", "<i4"),
+                     ("p", PART), ("nse", "<i4")])
+    se_dt = np.dtype([("p", PART), ("mat", "<i4")])
+    out = np.zeros(n, [("status", "<i4"), ("next", "<i4"), ("mat", "<i4"), ("last_tri", "<i4"),
+                       ("p", PART), ("nse", "<i4"), ("se", PART), ("se_mat", "<i4")])
+    for i in range(n):
+        h = raw[off:off + head.itemsize].view(head)[0]
+        off += head.itemsize
+        for f in head.names:
+            out[i][f] = h[f]
+        if h["nse"]:
+            s = raw[off:off + se_dt.itemsize].view(se_dt)[0]
+            off += se_dt.itemsize * h["nse"]
+            out[i]["se"] = s["p"]
+            out[i]["se_mat"] = s["mat"]
+    assert off == len(raw)
+    return out
+
+
+# Physics variants of the harness's stepx / simulatex commands: (name, model
+# of the material files, energy threshold in eV).
+VARIANTS = [("penn", "penn", 0.0), ("kieft", "kieft", 0.0), ("penn_abs", "penn", 0.0),
+            ("penn_alt", "penn", 2.0), ("kieft_alt", "kieft", 3.0), ("penn_thr", "penn", 1.5)]
+
+
+def special_states(n, seed):
+    """States in the vacuum above special_geometry's block (every special
+    plane within reach) and inside the block near its NOP plane; a quarter
+    of them at exactly 50 eV or one ulp either side (the LT50 / GE50 edge)."""
+    rng = np.random.default_rng(seed)
+    s = np.zeros(n, ST)
+    inside = rng.random(n) < 0.25
+    s["pos"][:, 0] = rng.uniform(-38, 38, n)
+    s["pos"][:, 1] = rng.uniform(-38, 38, n)
+    s["pos"][:, 2] = np.where(inside, -rng.uniform(0.05, 10.0, n), rng.uniform(0.05, 19.95, n))
+    d = rng.standard_normal((n, 3))
+    s["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    e = np.exp(rng.uniform(np.log(14.0), np.log(3000.0), n)).astype(np.float32)
+    edge = rng.random(n) < 0.25
+    f50 = np.float32(50.0)
+    choice = rng.integers(0, 3, n)
+    e50 = np.where(choice == 0, f50, np.where(choice == 1, np.nextafter(f50, np.float32(0)),
+                                              np.nextafter(f50, np.float32(100))))
+    s["E"] = np.where(edge, e50, e)
+    s["mat"] = np.where(inside, 0, -123)
+    s["last_tri"] = -1
+    return s
+
+
+def variants():
+    """reference_variants.npz: stepx / simulatex vectors."""
+    tmp = tempfile.mkdtemp()
+    fx = {}
+    spec_mesh = W.special_geometry()
+    grat = small_grating()
+    sf_tris = os.path.join(tmp, "special.tris")
+    gf_tris = os.path.join(tmp, "grating_small.tris")
+    tri_file(sf_tris, spec_mesh)
+    tri_file(gf_tris, grat)
+    fx["special_mesh_sha"] = sha(spec_mesh.vertices()) + sha(spec_mesh.mat_in) + sha(spec_mesh.mat_out)
+    fx["grating_small_mesh_sha"] = sha(grat.vertices()) + sha(grat.mat_in) + sha(grat.mat_out)
+    mats = {}
+    for model in ("penn", "kieft"):
+        for mname, spec in (("si", W.SI), ("pmma", W.PMMA)):
+            mf = os.path.join(tmp, f"{mname}_{model}.mat")
+            A.write_mat_file(mf, W.make_material(spec, model))
+            mats[(mname, model)] = mf
+    st_special = special_states(3000, seed=41)
+    st_grating = states_for(3000, seed=42)
+    fx["special_states"] = st_special
+    fx["grating_states"] = st_grating
+    pri = W.raster_scan(4, 4, 8, 400.0, -30, 30, -30, 30, 2.0, seed=11)
+    fx["special_pri"] = pri
+    pf = os.path.join(tmp, "pri.bin")
+    W.write_pri(pf, pri)
+    for name, model, thr in VARIANTS:
+        variant = "penn" if name == "penn_thr" else name
+        for geo, tf, st in (("special", sf_tris, st_special), ("grating", gf_tris, st_grating)):
+            if geo == "grating" and name in ("penn", "kieft"):
+                continue   # the default physics on the grating is in reference_vectors.npz
+            stf = os.path.join(tmp, "states.bin")
+            st.tofile(stf)
+            of = os.path.join(tmp, "steps.out")
+            run("stepx", tf, variant, thr, 4321, stf, of, mats[("si", model)], mats[("pmma", model)])
+            out = parse_steps(of, len(st))
+            fx[f"step_{name}_{geo}"] = out
+            print(f"stepx {name} {geo}: status {np.bincount(out['status'], minlength=5)}, "
+                  f"se {out['nse'].sum()}")
+        so = os.path.join(tmp, "sim.out")
+        run("simulatex", sf_tris, variant, thr, 91, pf, so, mats[("si", model)])
+        b = np.fromfile(so, np.uint8)
+        n = int(b[-8:].view("<i8")[0])
+        det = b[:-8].view([("p", PART), ("tag", "<u4")])
+        assert len(det) == n
+        fx[f"sim_{name}_special"] = det.copy()
+        print(f"simulatex {name}: {len(pri)} primaries, {n} detected")
+    out = os.path.join(OUT, "reference_variants.npz")
+    np.savez_compressed(out, **fx)
+    print("wrote", out, os.path.getsize(out), "bytes")
 
 
 def main():
@@ -268,4 +380,9 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "variants":
+        if not os.path.exists(HARNESS):
+            sys.exit("oracle/_ref/harness missing: run `make -C oracle/ref` first")
+        variants()
+    else:
+        main()

@@ -216,3 +216,60 @@ def test_energy_deposits_match_reference(model):
     assert np.array_equal(det["tag"], rdet["tag"])
     assert np.array_equal(bits(det["E"]), bits(rdet["p"]["E"]))
 
+
+
+# Bounds of the portable float math's continuous outputs against the
+# reference's C-library math on the same single steps, in units of
+# float epsilon (2^-23) times each quantity's scale: |dir|, the larger of
+# |pos| before and after, |E|, and for a secondary's energy the energy of
+# its parent before the event (the secondary's energy fermi + omega -
+# binding can cancel, so its own magnitude is no scale).  Measured: dir
+# 15.7 / 12.9, pos 0.9 / 0.8, E 1.0 / 5.6, secondary energy 0.8 / 5.2
+# (Penn / Kieft).
+PF_BOUNDS = {"dir": 32.0, "pos": 2.0, "E": 12.0, "se_dir": 24.0, "se_pos": 2.0, "se_E": 12.0}
+
+
+@pytest.mark.parametrize("model", ["penn", "kieft"])
+def test_portable_math_steps_match_reference_decisions(model):
+    """The GPU's transcendentals (the oracle's "pf" build, bit-identical to
+    the HIP kernels) replayed on the reference's single-step vectors: every
+    discrete outcome (status, event, material, last triangle, secondary
+    made) equals the reference's (scatter_list.h:143-170 event selection,
+    boundary_intersect.h:38-155 transmission choice) on all 6000 states,
+    and the continuous outputs stay within PF_BOUNDS."""
+    ow = _world(model, kind="pf")
+    st = FX[f"step_{model}_states"]
+    ref = FX[f"step_{model}_out"]
+    s = np.zeros(len(st), O.STATE_DTYPE)
+    for k in ("pos", "dir", "E", "mat", "last_tri"):
+        s[k] = st[k]
+    o_st, o_ev, o_oc, o_se, o_made = O.step_states_mt(ow, s, 1234)
+    status = np.where(o_oc == 1, 4, np.where(o_oc == 2, 3, np.where(o_ev == 3, 1, np.where(o_ev == 0, 2, 0))))
+    assert np.array_equal(status, ref["status"])
+    scat = ref["status"] == 0
+    assert np.array_equal(o_ev[scat], ref["next"][scat])
+    assert np.array_equal(o_st["mat"], ref["mat"])
+    assert np.array_equal(o_st["last_tri"], ref["last_tri"])
+    assert np.array_equal(o_made, ref["nse"])
+    eps = 2.0 ** -23
+
+    def err(a, b, scale):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        return float(np.max(np.abs(a - b) / scale)) / eps
+
+    def norm(v):
+        return np.linalg.norm(np.asarray(v, np.float64), axis=1)[:, None]
+
+    p = ref["p"]
+    assert err(o_st["dir"], p["dir"], norm(p["dir"])) <= PF_BOUNDS["dir"]
+    assert err(o_st["pos"], p["pos"], np.maximum(norm(p["pos"]), norm(st["pos"]))) <= PF_BOUNDS["pos"]
+    assert err(o_st["E"], p["E"], np.abs(p["E"].astype(np.float64))) <= PF_BOUNDS["E"]
+    m = o_made.astype(bool)
+    assert m.sum() > 100
+    se = ref["se"][m]
+    assert err(o_se["dir"][m], se["dir"], norm(se["dir"])) <= PF_BOUNDS["se_dir"]
+    assert err(o_se["pos"][m], se["pos"], norm(se["pos"])) <= PF_BOUNDS["se_pos"]
+    assert err(o_se["E"][m], se["E"], st["E"][m].astype(np.float64)) <= PF_BOUNDS["se_E"]
+    # the two maths do differ (this is not the "ref" build by mistake)
+    assert not np.array_equal(bits(o_st["dir"]), bits(p["dir"]))
