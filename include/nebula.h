@@ -229,7 +229,8 @@ int nbl_driver_push(nbl_driver* d, const nbl_particle28* particles, const uint32
  * material list (world 0 is the creation geometry) and returns its id;
  * push_world queues primaries that track in that world.  Secondaries stay
  * in their parent's world.  Physics flags, threshold and seed are shared;
- * nbl_driver_set_material_params applies to world 0's materials. */
+ * nbl_driver_set_material_params applies to the material of that index in
+ * every world that has one. */
 int nbl_driver_add_world(nbl_driver* d, const nbl_geometry* geom, const nbl_material* const* mats,
 	int32_t n_mats, int32_t* world);
 int nbl_driver_push_world(nbl_driver* d, const nbl_particle28* particles, const uint32_t* tags,
@@ -300,6 +301,21 @@ int nbl_driver_run(nbl_driver* d, nbl_detected* out, uint64_t cap, uint64_t* n_d
 /* The HIP stream the driver launches on (a hipStream_t). */
 void* nbl_driver_stream(nbl_driver* d);
 
+/* Introspection (no reference counterpart): the sizes of a driver's device
+ * buffers, for tests and capacity planning.  The primary queue keeps only
+ * entries not yet moved into the particle buffer when primaries are pushed
+ * (a drained queue starts again from 0); the spill stack of secondaries
+ * that found no room doubles whenever it is found more than half full. */
+typedef struct {
+	uint64_t queue_capacity;    /* primaries the device queue can hold     */
+	uint64_t queue_entries;     /* entries held (unread + read since reuse) */
+	uint32_t slots;             /* particle-buffer entries (2 x capacity)  */
+	uint32_t spill_capacity;    /* secondaries the spill stack can hold    */
+	uint32_t spill_growths;     /* doublings of the spill stack so far     */
+	uint32_t detected_capacity; /* detected records held between drains    */
+} nbl_driver_info;
+int nbl_driver_get_info(nbl_driver* d, nbl_driver_info* out);
+
 /* Measurement: time every tracking-kernel launch with HIP events on the
  * driver's stream (no reference counterpart; replaces time_log of
  * common/time_log.h for the kernel).  nbl_driver_timing waits for the stream
@@ -367,8 +383,9 @@ int nbl_driver_step_states(nbl_driver* d, nbl_step_state* states, int32_t* event
 #define NBL_KIND_BARRIER 2
 #define NBL_MAX_SCORE_PARAMS 8
 
-/* Model parameters of material `material` for this driver: free paths are
- * sampled with the mean free path x scale; `barrier` replaces the file's. */
+/* Model parameters of material `material` for this driver (in every world
+ * that has a material of that index): free paths are sampled with the mean
+ * free path x scale; `barrier` replaces the file's. */
 int nbl_driver_set_material_params(nbl_driver* d, int32_t material, float el_mfp_scale,
 	float in_mfp_scale, float barrier);
 int nbl_driver_get_material_params(nbl_driver* d, int32_t material, float* el_mfp_scale,

@@ -77,6 +77,12 @@ class TuneResult(C.Structure):
                 ("trials", C.c_uint32), ("steps_per_s", C.c_double)]
 
 
+class DriverInfo(C.Structure):
+    _fields_ = [("queue_capacity", C.c_uint64), ("queue_entries", C.c_uint64), ("slots", C.c_uint32),
+                ("spill_capacity", C.c_uint32), ("spill_growths", C.c_uint32),
+                ("detected_capacity", C.c_uint32)]
+
+
 class Table(C.Structure):
     _fields_ = [("data", C.POINTER(C.c_float)), ("ndim", C.c_int32), ("dims", C.c_int32 * 3),
                 ("min", C.c_float * 3), ("step", C.c_float * 3)]
@@ -154,6 +160,7 @@ _SIGS = {
     "nbl_driver_steps": (C.c_int, [C.c_void_p, _p(C.c_uint64)]),
     "nbl_driver_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, _p(C.c_uint64)]),
     "nbl_driver_stream": (C.c_void_p, [C.c_void_p]),
+    "nbl_driver_get_info": (C.c_int, [C.c_void_p, C.c_void_p]),
     "nbl_driver_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "nbl_driver_timing": (C.c_int, [C.c_void_p, _p(C.c_double), _p(C.c_uint64)]),
     "nbl_geometry_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -623,6 +623,12 @@ class Driver:
         check(lib().nbl_driver_steps(self._h, C.byref(s)))
         return s.value
 
+    def info(self) -> dict:
+        """Sizes of the driver's device buffers (nbl_driver_get_info)."""
+        r = _lib.DriverInfo()
+        check(lib().nbl_driver_get_info(self._h, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in r._fields_}
+
     def set_timing(self, enable: bool) -> None:
         check(lib().nbl_driver_set_timing(self._h, 1 if enable else 0))
 

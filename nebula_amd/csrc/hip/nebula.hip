@@ -799,6 +799,7 @@ struct nbl_driver {
 	uint64_t q_cap, q_n;
 	std::vector<WorldDesc> h_worlds;   // world 0 = the creation geometry
 	std::vector<Material*> world_mats; // device material arrays of worlds 1..
+	std::vector<std::vector<Material>> h_world_mats;   // their host copies
 	WorldDesc* d_worlds;
 	uint32_t next_tag;
 	ScoreMap map;           // map.np == 0: no scores
@@ -814,6 +815,7 @@ struct nbl_driver {
 	uint64_t epoch = 0;             // bumped when a launch argument changes
 	struct Graph { uint32_t n; uint64_t epoch; hipGraph_t g; hipGraphExec_t exec; };
 	std::vector<Graph> graphs;
+	uint32_t spill_growths = 0;     // doublings of the spill stack
 	bool timing;
 	std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_used, ev_free;
 	double step_ms;
@@ -1114,22 +1116,46 @@ int nbl_material_upload(nbl_device* dev, const nbl_material_host* m, nbl_materia
 	clear_error();
 	if (!dev || !m || !out)
 		return fail(NBL_ERR_ARG, "null argument");
+	if (m->inel_model != NBL_INEL_PENN && m->inel_model != NBL_INEL_KIEFT)
+		return fail(NBL_ERR_ARG, "unknown inelastic model " + std::to_string(m->inel_model));
+	// Every table the kernels read must be present with the dimensions its
+	// lookups assume (at least two entries per axis: the interpolation reads
+	// index n - 2 and n - 1), so that no lookup reads outside it.
+	const nbl_table* ts[7] = { &m->el_imfp, &m->el_icdf, &m->in_imfp, &m->in_icdf, &m->pn_q, &m->pn_ion, &m->ki_ion };
+	static const char* const names[7] = { "elastic imfp", "elastic cos(theta) icdf", "inelastic imfp",
+		"inelastic icdf", "Penn q icdf", "Penn binding icdf", "Kieft binding icdf" };
+	const bool penn = m->inel_model == NBL_INEL_PENN;
+	const int want[7] = { 1, 2, 1, 2, penn ? 3 : 0, penn ? 3 : 0, penn ? 0 : 2 };
+	for (int k = 0; k < 7; ++k) {
+		if (!want[k])
+			continue;
+		const nbl_table& t = *ts[k];
+		if (!t.data || t.ndim != want[k])
+			return fail(NBL_ERR_FORMAT, std::string(names[k]) + " table missing or not " +
+				std::to_string(want[k]) + "-D");
+		for (int a = 0; a < t.ndim; ++a)
+			if (t.dims[a] < 2)
+				return fail(NBL_ERR_FORMAT, std::string(names[k]) + " table needs at least 2 entries per axis");
+	}
 	int rc = set_device(dev);
 	if (rc) return rc;
 	auto* mm = new nbl_material();
 	mm->dev = dev;
-	const nbl_table* ts[7] = { &m->el_imfp, &m->el_icdf, &m->in_imfp, &m->in_icdf, &m->pn_q, &m->pn_ion, &m->ki_ion };
+	auto release = [&](int code) {
+		for (float*& t : mm->tables) { hipFree(t); t = nullptr; }
+		delete mm;
+		return code;
+	};
 	for (int k = 0; k < 7; ++k) {
-		const size_t c = tcount(*ts[k]);
+		const size_t c = want[k] ? tcount(*ts[k]) : 0;
 		mm->tables[k] = nullptr;
 		if (!c)
 			continue;
-		if ((rc = dev_alloc(&mm->tables[k], c))) {
-			for (int j = 0; j < k; ++j) hipFree(mm->tables[j]);
-			delete mm;
-			return rc;
-		}
-		HIP_OK(hipMemcpy(mm->tables[k], ts[k]->data, sizeof(float) * c, hipMemcpyHostToDevice));
+		if ((rc = dev_alloc(&mm->tables[k], c)))
+			return release(rc);
+		const hipError_t e = hipMemcpy(mm->tables[k], ts[k]->data, sizeof(float) * c, hipMemcpyHostToDevice);
+		if (e != hipSuccess)
+			return release(fail(NBL_ERR_DEVICE, std::string("material table upload: ") + hipGetErrorString(e)));
 	}
 	Material& d = mm->m;
 	d.barrier = m->barrier;
@@ -1147,10 +1173,6 @@ int nbl_material_upload(nbl_device* dev, const nbl_material_host* m, nbl_materia
 	d.pn_q = tab3(m->pn_q, mm->tables[4]);
 	d.pn_ion = tab3(m->pn_ion, mm->tables[5]);
 	d.ki_ion = tab2(m->ki_ion, mm->tables[6]);
-	if (m->inel_model == NBL_INEL_PENN && (!mm->tables[4] || !mm->tables[5]))
-		return fail(NBL_ERR_FORMAT, "full Penn material without q / binding tables");
-	if (m->inel_model == NBL_INEL_KIEFT && !mm->tables[6])
-		return fail(NBL_ERR_FORMAT, "Kieft material without binding table");
 	*out = mm;
 	return NBL_OK;
 }
@@ -1309,6 +1331,7 @@ int nbl_driver_add_world(nbl_driver* d, const nbl_geometry* geom, const nbl_mate
 		return fail(NBL_ERR_DEVICE, "material descriptor upload failed");
 	}
 	d->world_mats.push_back(dm);
+	d->h_world_mats.push_back(hm);
 	WorldDesc w;
 	w.g = geom->g;
 	w.mats = dm;
@@ -1337,24 +1360,47 @@ int nbl_driver_push_world(nbl_driver* d, const nbl_particle28* particles, const 
 		return fail(NBL_ERR_ARG, "no such world " + std::to_string(world));
 	int rc = set_device(d->dev);
 	if (rc) return rc;
-	HIP_OK(hipStreamSynchronize(d->stream));
-	// device queue: grow by doubling, keeping the unread part
-	if (d->q_n + n > d->q_cap) {
-		uint64_t nc = std::max<uint64_t>(d->q_cap ? 2 * d->q_cap : 1024, d->q_n + n);
-		nbl_particle28* np; uint32_t* nt; uint64_t* ns; uint16_t* nw;
-		if ((rc = dev_alloc(&np, nc)) || (rc = dev_alloc(&nt, nc)) || (rc = dev_alloc(&ns, nc)) ||
-			(rc = dev_alloc(&nw, nc)))
-			return rc;
-		if (d->q_n) {
-			HIP_OK(hipMemcpy(np, d->q_part, sizeof(nbl_particle28) * d->q_n, hipMemcpyDeviceToDevice));
-			HIP_OK(hipMemcpy(nt, d->q_tag, sizeof(uint32_t) * d->q_n, hipMemcpyDeviceToDevice));
-			HIP_OK(hipMemcpy(ns, d->q_sid, sizeof(uint64_t) * d->q_n, hipMemcpyDeviceToDevice));
-			HIP_OK(hipMemcpy(nw, d->q_world, sizeof(uint16_t) * d->q_n, hipMemcpyDeviceToDevice));
+	// The stream is idle after this read, so the queue can be rearranged.
+	if ((rc = read_ctrl(d)))
+		return rc;
+	// Entries before queue_next have been moved into the particle buffer.
+	// Only the unread part [queue_next, q_n) is kept: once the queue is
+	// drained it starts again from 0, and growth or compaction copies the
+	// unread part alone to the front, so repeated push / run cycles reuse
+	// the same memory.
+	const uint64_t next = std::min<uint64_t>(d->h_ctrl->queue_next, d->q_n);
+	const uint64_t unread = d->q_n - next;
+	if (d->q_n + n > d->q_cap || (next > 0 && unread == 0)) {
+		const uint64_t need = unread + n;
+		uint64_t nc = d->q_cap;
+		if (need > d->q_cap)
+			nc = std::max<uint64_t>(d->q_cap ? 2 * d->q_cap : 1024, need);
+		if (unread > 0 || nc != d->q_cap) {
+			// a fresh allocation (the unread part may overlap its new place)
+			nbl_particle28* np; uint32_t* nt; uint64_t* ns; uint16_t* nw;
+			np = nullptr; nt = nullptr; ns = nullptr; nw = nullptr;
+			if ((rc = dev_alloc(&np, nc)) || (rc = dev_alloc(&nt, nc)) || (rc = dev_alloc(&ns, nc)) ||
+				(rc = dev_alloc(&nw, nc))) {
+				hipFree(np); hipFree(nt); hipFree(ns); hipFree(nw);
+				return rc;
+			}
+			if (unread) {
+				HIP_OK(hipMemcpy(np, d->q_part + next, sizeof(nbl_particle28) * unread, hipMemcpyDeviceToDevice));
+				HIP_OK(hipMemcpy(nt, d->q_tag + next, sizeof(uint32_t) * unread, hipMemcpyDeviceToDevice));
+				HIP_OK(hipMemcpy(ns, d->q_sid + next, sizeof(uint64_t) * unread, hipMemcpyDeviceToDevice));
+				HIP_OK(hipMemcpy(nw, d->q_world + next, sizeof(uint16_t) * unread, hipMemcpyDeviceToDevice));
+			}
+			hipFree(d->q_part); hipFree(d->q_tag); hipFree(d->q_sid); hipFree(d->q_world);
+			d->q_part = np; d->q_tag = nt; d->q_sid = ns; d->q_world = nw;
+			d->epoch++;
+			d->q_cap = nc;
 		}
-		hipFree(d->q_part); hipFree(d->q_tag); hipFree(d->q_sid); hipFree(d->q_world);
-		d->q_part = np; d->q_tag = nt; d->q_sid = ns; d->q_world = nw;
-		d->epoch++;
-		d->q_cap = nc;
+		d->q_n = unread;
+		const uint64_t qv[2] = { unread, 0 };   // queue_n, queue_next
+		static_assert(offsetof(Ctrl, queue_next) == offsetof(Ctrl, queue_n) + 8, "queue words adjacent");
+		HIP_OK(hipMemcpy(&d->ctrl->queue_n, qv, sizeof(qv), hipMemcpyHostToDevice));
+		d->h_ctrl->queue_n = unread;
+		d->h_ctrl->queue_next = 0;
 	}
 	std::vector<uint32_t> t(n);
 	std::vector<uint64_t> s(n);
@@ -1749,6 +1795,7 @@ int grow_spill(nbl_driver* d)
 		d->sc_mem[2] = nsc;
 	}
 	d->spill_cap = ncap;
+	d->spill_growths++;
 	d->epoch++;
 	return NBL_OK;
 }
@@ -1948,12 +1995,25 @@ int nbl_driver_set_material_params(nbl_driver* d, int32_t material, float el_mfp
 		return fail(NBL_ERR_ARG, "mean-free-path scales must be positive and finite, the barrier finite");
 	int rc = set_device(d->dev);
 	if (rc) return rc;
-	Material& m = d->h_mats[(size_t)material];
-	m.el_scale = el_mfp_scale;
-	m.in_scale = in_mfp_scale;
-	m.barrier = barrier;
-	HIP_OK(hipMemcpyAsync(d->d_mats + material, &m, sizeof(Material), hipMemcpyHostToDevice, d->stream));
+	// the material arrays are read by kernels in flight: change them when idle
 	HIP_OK(hipStreamSynchronize(d->stream));
+	auto apply = [&](Material& m) {
+		m.el_scale = el_mfp_scale;
+		m.in_scale = in_mfp_scale;
+		m.barrier = barrier;
+	};
+	apply(d->h_mats[(size_t)material]);
+	HIP_OK(hipMemcpy(d->d_mats + material, &d->h_mats[(size_t)material], sizeof(Material), hipMemcpyHostToDevice));
+	// the same material index of every other world (the scans of a batch
+	// share their material list)
+	for (size_t w = 0; w < d->world_mats.size(); ++w) {
+		std::vector<Material>& hm = d->h_world_mats[w];
+		if ((size_t)material >= hm.size() || (int)w + 1 >= (int)d->h_worlds.size() ||
+			material >= d->h_worlds[w + 1].nmats)
+			continue;
+		apply(hm[(size_t)material]);
+		HIP_OK(hipMemcpy(d->world_mats[w] + material, &hm[(size_t)material], sizeof(Material), hipMemcpyHostToDevice));
+	}
 	return NBL_OK;
 }
 
@@ -2016,6 +2076,20 @@ int nbl_driver_set_score_params(nbl_driver* d, const int32_t* materials, const i
 }
 
 void* nbl_driver_stream(nbl_driver* d) { return d ? (void*)d->stream : nullptr; }
+
+int nbl_driver_get_info(nbl_driver* d, nbl_driver_info* out)
+{
+	clear_error();
+	if (!d || !out)
+		return fail(NBL_ERR_ARG, "null argument");
+	out->queue_capacity = d->q_cap;
+	out->queue_entries = d->q_n;
+	out->slots = d->slots;
+	out->spill_capacity = d->spill_cap;
+	out->spill_growths = d->spill_growths;
+	out->detected_capacity = d->det_cap;
+	return NBL_OK;
+}
 
 int nbl_geometry_query(const nbl_geometry* g, const float* pos, const float* dir, const float* dist,
 	const int32_t* ignore_tri, const int32_t* ignore_mat, int32_t* out_tri, float* out_dist, uint32_t n)

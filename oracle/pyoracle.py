@@ -337,7 +337,11 @@ class OracleWorld:
         if octree is None:
             octree = build_octree(tris, kind)
         self.octree_data, self.triangles, self.center, self.halfsize = octree
-        self.mats = [OracleMaterial(r, kind) for r in raw_materials]
+        # raw materials, or materials already prepared by this build (shared
+        # between worlds)
+        self.mats = [r if isinstance(r, OracleMaterial) else OracleMaterial(r, kind) for r in raw_materials]
+        if any(m.L is not self.L for m in self.mats):
+            raise ValueError("materials prepared by the other oracle build")
         self.mat_arr = (Material * max(1, len(self.mats)))(*[m.m for m in self.mats])
         w = World()
         g = w.geom

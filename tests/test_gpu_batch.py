@@ -105,14 +105,63 @@ def test_autotune_keeps_results(dev):
 def test_batched_scans_equal_plain_scans(dev):
     """A reduced version of the 64-scan batch (6 scans, uneven sizes and
     energies, gratings and contact holes): the batched run (worlds, graphs,
+    steps per launch followed online, and once more with the one-off
     autotuned launch) and the scans run one by one give identical per-scan
     SE and BSE counts."""
     from nebula_amd import batch as B
     scans = W.batch_scans(6, seed=3, per_pixel_scale=0.02)
     mats = [A.Material(dev, A.HostMaterial(material(m))) for m in ("si", "pmma")]
     cb, tb, tune = B.run_batched(dev, scans, mats, capacity=1 << 18, seed=2, iters_per_trial=2)
+    ca, ta, tune_a = B.run_batched(dev, scans, mats, capacity=1 << 18, seed=2, iters_per_trial=2,
+                                   autotune=True, adapt_k=False)
     cp, tp = B.run_plain(dev, scans, mats, capacity=1 << 18, seed=2)
     n = sum(len(s["primaries"]) for s in scans)
-    print(f"batch of 6: {n} primaries, batched {tb:.3f} s, plain {tp:.3f} s, tune {tune}")
+    print(f"batch of 6: {n} primaries, batched {tb:.3f} s, autotuned {ta:.3f} s ({tune_a}), plain {tp:.3f} s")
     assert np.all(cb.sum(axis=1) > 0)
     assert np.array_equal(cb, cp)
+    assert np.array_equal(ca, cp)
+    assert tune_a["trials"] >= 3
+
+
+def test_material_params_apply_to_every_world(dev):
+    """nbl_driver_set_material_params changes the material of that index in
+    every world of a batched driver: both worlds' records equal the
+    oracle's with the changed parameters."""
+    si = material("si")
+    g = A.Geometry(dev, slab_tris())
+    m_si = A.Material(dev, A.HostMaterial(si))
+    d = A.Driver(dev, 1024, g, [m_si], seed=19, steps_per_launch=8)
+    w1 = d.add_world(A.Geometry(dev, slab_tris()), [m_si])
+    d.set_material_params(0, el_mfp_scale=1.3, in_mfp_scale=0.8, barrier=11.0)
+    p = W.pencil_beam(60, 1000.0)
+    d.push(p, np.arange(60, dtype=np.uint32), sid_base=0, world=0)
+    d.push(p, np.arange(60, 120, dtype=np.uint32), sid_base=1 << 32, world=w1)
+    got = d.run(100000)
+    ow = O.OracleWorld(slab_tris(), [si])
+    ow.set_params(0, el_scale=1.3, in_scale=0.8, barrier=11.0)
+    e0, _ = ow.run_philox(p, 19, 100000, threads=8, sid_base=0, tag_base=0)
+    e1, _ = ow.run_philox(p, 19, 100000, threads=8, sid_base=1 << 32, tag_base=60)
+    exp = np.concatenate([e0, e1])
+    assert len(e0) > 10 and len(got) == len(exp)
+    assert _keyed(got).tobytes() == _keyed(exp).tobytes()
+
+
+def test_material_upload_refuses_incomplete_tables(dev):
+    """A hand-built host material whose tables are missing or too small is
+    refused at upload (NBL_ERR_FORMAT) instead of giving kernels
+    out-of-bounds reads."""
+    for model, name, field, value in (("penn", "el_icdf", "ndim", 1), ("penn", "pn_q", "data", None),
+                                      ("kieft", "ki_ion", "dims", 1), ("penn", "in_imfp", "dims", 1)):
+        hm = A.HostMaterial(material("si", model))
+        t = getattr(hm.s, name)
+        saved = (t.ndim, t.data, t.dims[0], t.dims[1])
+        if field == "ndim":
+            t.ndim = value
+        elif field == "data":
+            t.data = None
+        else:
+            t.dims[t.ndim - 1] = value
+        with pytest.raises(A.NebulaError, match="table"):
+            A.Material(dev, hm)
+        t.ndim, t.data, t.dims[0], t.dims[1] = saved
+        A.Material(dev, hm)   # the restored material uploads
