@@ -302,9 +302,11 @@ int nbl_driver_run(nbl_driver* d, nbl_detected* out, uint64_t cap, uint64_t* n_d
 void* nbl_driver_stream(nbl_driver* d);
 
 /* Introspection (no reference counterpart): the sizes of a driver's device
- * buffers, for tests and capacity planning.  The primary queue keeps only
- * entries not yet moved into the particle buffer when primaries are pushed
- * (a drained queue starts again from 0); the spill stack of secondaries
+ * buffers, for tests and capacity planning.  A push appends behind the
+ * queue's entries when it fits; otherwise only the entries not yet moved
+ * into the particle buffer are kept, moved to the front (a drained queue
+ * starts again from 0), and the queue grows only when those and the new
+ * primaries exceed its capacity.  The spill stack of secondaries
  * that found no room doubles whenever it is found more than half full. */
 typedef struct {
 	uint64_t queue_capacity;    /* primaries the device queue can hold     */

@@ -3,6 +3,8 @@
 Every particle keeps its world through its cascade, and its trajectory is
 a function of (seed, stream id) only, so a mixed run must return exactly
 the union of the oracle's per-world runs (records keyed by stream id)."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -112,7 +114,8 @@ def test_batched_scans_equal_plain_scans(dev):
     scans = W.batch_scans(6, seed=3, per_pixel_scale=0.02)
     mats = [A.Material(dev, A.HostMaterial(material(m))) for m in ("si", "pmma")]
     cb, tb, tune = B.run_batched(dev, scans, mats, capacity=1 << 18, seed=2, iters_per_trial=2)
-    ca, ta, tune_a = B.run_batched(dev, scans, mats, capacity=1 << 18, seed=2, iters_per_trial=2,
+    # a smaller population, so that the queue outlasts the tuner's ramp
+    ca, ta, tune_a = B.run_batched(dev, scans, mats, capacity=1 << 16, seed=2, iters_per_trial=1,
                                    autotune=True, adapt_k=False)
     cp, tp = B.run_plain(dev, scans, mats, capacity=1 << 18, seed=2)
     n = sum(len(s["primaries"]) for s in scans)
@@ -154,7 +157,8 @@ def test_material_upload_refuses_incomplete_tables(dev):
                                       ("kieft", "ki_ion", "dims", 1), ("penn", "in_imfp", "dims", 1)):
         hm = A.HostMaterial(material("si", model))
         t = getattr(hm.s, name)
-        saved = (t.ndim, t.data, t.dims[0], t.dims[1])
+        # the address, not the pointer field itself (which aliases the struct)
+        saved = (t.ndim, C.cast(t.data, C.c_void_p).value, t.dims[0], t.dims[1])
         if field == "ndim":
             t.ndim = value
         elif field == "data":
@@ -163,5 +167,6 @@ def test_material_upload_refuses_incomplete_tables(dev):
             t.dims[t.ndim - 1] = value
         with pytest.raises(A.NebulaError, match="table"):
             A.Material(dev, hm)
-        t.ndim, t.data, t.dims[0], t.dims[1] = saved
+        t.ndim, t.dims[0], t.dims[1] = saved[0], saved[2], saved[3]
+        t.data = C.cast(C.c_void_p(saved[1]), C.POINTER(C.c_float))
         A.Material(dev, hm)   # the restored material uploads

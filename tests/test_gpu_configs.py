@@ -236,8 +236,9 @@ def test_spill_stack_doubles_and_results_hold(dev, graph):
 def test_primary_queue_is_reused(dev):
     """Push / run cycles on one driver reuse the device queue (its capacity
     stays that of one cycle's push), and a push while primaries of an
-    earlier push are still queued keeps them: the unread part moves to the
-    front.  Every cycle's records equal the oracle's."""
+    earlier push are still queued keeps them: appended behind them when it
+    fits, else with the unread part moved to the front.  Every cycle's
+    records equal the oracle's."""
     tris = slab_tris()
     raw = material("si")
     seed = 17
@@ -257,15 +258,24 @@ def test_primary_queue_is_reused(dev):
         n_common, n_same, n_only = _compare_detected(det_g, det_o)
         assert n_only == 0 and n_common == n_same == len(det_o) == len(det_g), cycle
     assert len(set(caps)) == 1, caps
-    # a push while the queue still holds unread primaries: run a few
-    # iterations of the first half, push the second half, run to the end
-    a, b = pri[:10000], pri[10000:]
+    # pushes while the queue still holds unread primaries: one that fits
+    # behind them is appended in place; one that does not moves the unread
+    # part to the front of memory of the same size (no growth)
+    a, b, c = pri[:10000], pri[10000:], pri[:1000]
     d.push(a, sid_base=0, tags=np.arange(10000, dtype=np.uint32))
     d.do_iteration(2)
-    running, _, queued = d.counts()
+    _, _, queued = d.counts()
     assert queued > 0
     d.push(b, sid_base=10000, tags=np.arange(10000, 20000, dtype=np.uint32))
-    assert d.info()["queue_entries"] == queued + len(b)
+    assert d.info()["queue_entries"] == 20000
+    d.do_iteration(2)
+    _, _, queued = d.counts()
+    assert 0 < queued and queued + len(c) <= caps[0]
+    d.push(c, sid_base=20000, tags=np.arange(20000, 21000, dtype=np.uint32))
+    info = d.info()
+    assert info["queue_entries"] == queued + len(c) and info["queue_capacity"] == caps[0], (info, queued)
     det_g = d.run(10 ** 6)
-    n_common, n_same, n_only = _compare_detected(det_g, det_o)
-    assert n_only == 0 and n_common == n_same == len(det_o) == len(det_g)
+    det_c, _ = ow.run_philox(c, seed, 10 ** 6, threads=8, sid_base=20000, tag_base=20000)
+    exp = np.concatenate([det_o, det_c])
+    n_common, n_same, n_only = _compare_detected(det_g, exp)
+    assert n_only == 0 and n_common == n_same == len(exp) == len(det_g)
