@@ -301,6 +301,30 @@ int nbl_driver_run(nbl_driver* d, nbl_detected* out, uint64_t cap, uint64_t* n_d
 /* The HIP stream the driver launches on (a hipStream_t). */
 void* nbl_driver_stream(nbl_driver* d);
 
+/* Tracking without allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code between a graph's capture calls,
+ * serving loops, harnesses that forbid hipMalloc).  The same tracking as
+ * nbl_driver_create + nbl_driver_push + nbl_driver_run with the plain
+ * kernels: primaries[i] (DEVICE array, n entries) gets tag tags[i] (device
+ * array, or NULL: tag i) and stream id sid_base + i; every detected electron
+ * of the cascades is written to `out` (DEVICE array of out_cap records, in
+ * detection order); *n_detected and *steps (particle-steps) are host
+ * outputs.  The library allocates nothing: particle buffers, inelastic
+ * queue, spill stack, control block and material descriptors live in one
+ * fixed device array of 49 MiB (51 398 656 bytes) per GPU, held until the process ends and
+ * shared by every call on that GPU (calls that overlap on one GPU race).
+ * Limits: capacity (particles in flight) in [64, 65536], at most 64
+ * materials (NBL_ERR_CAPACITY otherwise); the spill stack holds 2^19
+ * secondaries and cannot grow, so a run that fills it with particles
+ * waiting for secondary slots ends with NBL_ERR_CAPACITY, as does one that
+ * detects more than out_cap electrons (*n_detected = out_cap then).
+ * Launches on the device's stream and returns when the run has ended.
+ * No scores, deposits or several worlds. */
+int nbl_track_noalloc(nbl_device* dev, const nbl_geometry* geom, const nbl_material* const* mats,
+	int32_t n_mats, uint32_t physics_flags, float energy_threshold, uint64_t seed, uint32_t capacity,
+	uint32_t steps_per_launch, const nbl_particle28* primaries, const uint32_t* tags, uint32_t n,
+	uint64_t sid_base, nbl_detected* out, uint32_t out_cap, uint32_t* n_detected, uint64_t* steps);
+
 /* Introspection (no reference counterpart): the sizes of a driver's device
  * buffers, for tests and capacity planning.  A push appends behind the
  * queue's entries when it fits; otherwise only the entries not yet moved

@@ -658,3 +658,20 @@ class Driver:
             self.close()
         except Exception:
             pass
+
+
+def track_noalloc(device: Device, geometry: Geometry, materials, primaries_dev: int, n: int, out_dev: int,
+                  out_cap: int, tags_dev: int | None = None, sid_base: int = 0,
+                  physics: int = _lib.NBL_PHYS_DEFAULT, energy_threshold: float = 0.0,
+                  seed: int = 0x78C7E39B, capacity: int = 1 << 16, steps_per_launch: int = 4):
+    """nbl_track_noalloc: track n primaries (28-byte records at DEVICE
+    address primaries_dev) and their cascades with no allocation, writing
+    the detected electrons (40-byte records) to DEVICE address out_dev.
+    Returns (records written, particle-steps)."""
+    arr = (C.c_void_p * max(1, len(materials)))(*[m._h for m in materials])
+    nd, steps = C.c_uint32(), C.c_uint64()
+    check(lib().nbl_track_noalloc(device._h, geometry._h, arr, len(materials), physics, energy_threshold, seed,
+                                  capacity, steps_per_launch, C.c_void_p(primaries_dev),
+                                  None if tags_dev is None else C.c_void_p(tags_dev), n, sid_base,
+                                  C.c_void_p(out_dev), out_cap, C.byref(nd), C.byref(steps)))
+    return nd.value, steps.value

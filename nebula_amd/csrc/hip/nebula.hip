@@ -274,10 +274,12 @@ __global__ void k_ctrl(Ctrl* c, uint32_t slots, uint32_t spill_cap)
 
 // Move planned primaries from the queue into the particle buffer
 // (gpu_driver.inl:367-389 / gpu_particle_manager.inl:342-353: vacuum
-// material, no last triangle).
+// material, no last triangle).  Without a tag array a primary's tag is its
+// queue index; without a stream-id array its stream id is sid_base + index.
 __global__ __launch_bounds__(256) void k_refill(Ctrl* c, Bufs b0, Bufs b1, Bufs spill,
 	const nbl_particle28* __restrict__ q, const uint32_t* __restrict__ qtag,
-	const uint64_t* __restrict__ qsid, const uint16_t* __restrict__ qworld, uint32_t np)
+	const uint64_t* __restrict__ qsid, const uint16_t* __restrict__ qworld, uint32_t np,
+	uint64_t sid_base)
 {
 	const uint32_t n = c->refill_n;
 	const uint32_t stride = gridDim.x * blockDim.x;
@@ -300,10 +302,10 @@ __global__ __launch_bounds__(256) void k_refill(Ctrl* c, Bufs b0, Bufs b1, Bufs 
 	s.p.E = p.kin_energy;
 	s.mat = MAT_VACUUM;
 	s.last_tri = -1;
-	s.tag = qtag[src];
+	s.tag = qtag ? qtag[src] : (uint32_t)src;
 	s.step = 0;
 	s.flags = qworld ? ((uint32_t)qworld[src] << FL_WORLD_SHIFT) : 0u;
-	s.sid = qsid[src];
+	s.sid = qsid ? qsid[src] : sid_base + src;
 	store_state(b, dst, s);
 	if (b.sc)
 		for (uint32_t q = 0; q < np; ++q)
@@ -754,10 +756,49 @@ __global__ __launch_bounds__(256) void k_step_states(World W, nbl_step_state* st
 }
 
 // ------------------------------------------------------- host objects --
+// ------------------------------------------ tracking without allocation --
+// nbl_track_noalloc keeps its particle buffers, inelastic queue, spill stack,
+// control block and material descriptors in one file-scope device array,
+// `scratch`, instead of allocating them.  Each GPU holds its own copy (at
+// its own address, looked up once per device when the device is opened)
+// until the process ends; every call on that GPU reuses it, so calls that
+// overlap on one GPU race.  Its size is fixed by the largest population it
+// serves: NA_MAX_CAPACITY particles in flight (2 x that many buffer slots),
+// NA_SPILL parked secondaries and NA_MAX_MATERIALS materials.
+constexpr uint32_t NA_MAX_CAPACITY = 1u << 16;
+constexpr uint32_t NA_SLOTS = 2u * NA_MAX_CAPACITY;
+constexpr uint32_t NA_SPILL = 1u << 19;
+constexpr int NA_MAX_MATERIALS = 64;
+constexpr size_t na_round(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t na_bufs_bytes(size_t n) { return na_round(n * (8 + 4 * N_FIELDS)); }
+constexpr size_t NA_OFF_MATS = na_round(sizeof(Ctrl));
+constexpr size_t NA_OFF_BUF0 = NA_OFF_MATS + na_round(sizeof(Material) * NA_MAX_MATERIALS);
+constexpr size_t NA_OFF_BUF1 = NA_OFF_BUF0 + na_bufs_bytes(NA_SLOTS);
+constexpr size_t NA_OFF_INEL = NA_OFF_BUF1 + na_bufs_bytes(NA_SLOTS);
+constexpr size_t NA_OFF_SPILL = NA_OFF_INEL + na_bufs_bytes(NA_SLOTS);
+constexpr size_t NA_BYTES = NA_OFF_SPILL + na_bufs_bytes(NA_SPILL);
+static_assert(NA_BYTES < (64u << 20), "the fixed tracking array stays below 64 MiB");
+
+static __device__ __align__(256) char scratch[NA_BYTES];
+
+// Start a run in the fixed array: a zeroed control block with the
+// population target and the number of queued primaries (a kernel rather
+// than a host copy, so nothing is staged in host memory).
+__global__ void k_na_init(Ctrl* c, uint32_t target, uint64_t queue_n)
+{
+	if (threadIdx.x != 0 || blockIdx.x != 0)
+		return;
+	Ctrl z = {};
+	z.target = target;
+	z.queue_n = queue_n;
+	*c = z;
+}
+
 struct nbl_device {
 	int ordinal;
 	int n_cu;           // compute units, for persistent grids
 	hipStream_t stream;
+	char* scratch;      // this device's address of `scratch`
 };
 
 struct nbl_geometry {
@@ -934,9 +975,16 @@ int nbl_device_open(int ordinal, nbl_device** out)
 	HIP_OK(hipSetDevice(ordinal));
 	int n_cu = 0;
 	HIP_OK(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ordinal));
+	// the address of the fixed tracking array on this device (host code
+	// uses it, never the array's name, wherever it needs that memory)
+	void* sp = nullptr;
+	HIP_OK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+	if (!sp)
+		return fail(NBL_ERR_DEVICE, "hipGetSymbolAddress returned no address for the tracking array");
 	auto* d = new nbl_device;
 	d->ordinal = ordinal;
 	d->n_cu = n_cu > 0 ? n_cu : 1;
+	d->scratch = (char*)sp;
 	hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
 	if (e != hipSuccess) {
 		delete d;
@@ -1483,7 +1531,7 @@ int enqueue_iteration(nbl_driver* d, const StepArgs& a)
 	const uint32_t g_step = std::min<uint32_t>(blocks_for(d->capacity), d->blocks_per_cu * (uint32_t)d->dev->n_cu);
 	hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(64), 0, d->stream, d->ctrl, d->slots, d->spill_cap);
 	hipLaunchKernelGGL(k_refill, dim3(g_refill), dim3(256), 0, d->stream, d->ctrl, d->buf[0],
-		d->buf[1], d->spill, d->q_part, d->q_tag, d->q_sid, d->q_world, (uint32_t)d->map.np);
+		d->buf[1], d->spill, d->q_part, d->q_tag, d->q_sid, d->q_world, (uint32_t)d->map.np, 0ull);
 	std::pair<hipEvent_t, hipEvent_t> ev{};
 	if (d->timing) {
 		if (d->ev_free.empty()) {
@@ -2262,6 +2310,116 @@ int nbl_driver_timing(nbl_driver* d, double* step_ms_total, uint64_t* step_launc
 	if (rc) return rc;
 	*step_ms_total = d->step_ms;
 	*step_launches = d->step_launches;
+	return NBL_OK;
+}
+
+// Track n primaries (device array) and their cascades to the end with the
+// buffers in the fixed device array: no allocation of any kind.
+int nbl_track_noalloc(nbl_device* dev, const nbl_geometry* geom, const nbl_material* const* mats,
+	int32_t n_mats, uint32_t physics_flags, float energy_threshold, uint64_t seed, uint32_t capacity,
+	uint32_t steps_per_launch, const nbl_particle28* primaries, const uint32_t* tags, uint32_t n,
+	uint64_t sid_base, nbl_detected* out, uint32_t out_cap, uint32_t* n_detected, uint64_t* steps)
+{
+	clear_error();
+	if (!dev || !geom || !n_detected || (n && !primaries) || (out_cap && !out) || n_mats < 0 ||
+		(n_mats > 0 && !mats))
+		return fail(NBL_ERR_ARG, "invalid nbl_track_noalloc arguments");
+	*n_detected = 0;
+	if (steps)
+		*steps = 0;
+	if (capacity < 64 || capacity > NA_MAX_CAPACITY)
+		return fail(NBL_ERR_CAPACITY, "nbl_track_noalloc: capacity must be in [64, " +
+			std::to_string(NA_MAX_CAPACITY) + "] (its buffers are a fixed device array)");
+	if (n_mats > NA_MAX_MATERIALS)
+		return fail(NBL_ERR_CAPACITY, "nbl_track_noalloc: at most " + std::to_string(NA_MAX_MATERIALS) +
+			" materials");
+	if (steps_per_launch < 1 || steps_per_launch > 1024)
+		return fail(NBL_ERR_ARG, "steps_per_launch must be in [1, 1024]");
+	if (geom->dev->ordinal != dev->ordinal)
+		return fail(NBL_ERR_ARG, "geometry on another device");
+	for (int i = 0; i < n_mats; ++i)
+		if (!mats[i] || mats[i]->dev->ordinal != dev->ordinal)
+			return fail(NBL_ERR_ARG, "material " + std::to_string(i) + " missing or on another device");
+	if (geom->max_material >= n_mats)
+		return fail(NBL_ERR_ARG, "not enough materials provided for this geometry: expected " +
+			std::to_string(geom->max_material + 1) + ", got " + std::to_string(n_mats));
+	int rc = set_device(dev);
+	if (rc) return rc;
+	if (n == 0)
+		return NBL_OK;
+	char* base = dev->scratch;
+	Ctrl* ctrl = (Ctrl*)base;
+	Material* dmats = (Material*)(base + NA_OFF_MATS);
+	const hipStream_t st = dev->stream;
+	for (int i = 0; i < n_mats; ++i)
+		HIP_OK(hipMemcpyAsync(dmats + i, &mats[i]->m, sizeof(Material), hipMemcpyHostToDevice, st));
+	const uint32_t slots = 2u * capacity;
+	StepArgs a = {};
+	a.W.g = geom->g;
+	a.W.mats = dmats;
+	a.W.nmats = n_mats;
+	a.W.flags = physics_flags;
+	a.W.threshold = energy_threshold;
+	a.W.key0 = (uint32_t)seed;
+	a.W.key1 = (uint32_t)(seed >> 32);
+	a.worlds = nullptr;
+	a.buf[0] = { base + NA_OFF_BUF0, nullptr, slots };
+	a.buf[1] = { base + NA_OFF_BUF1, nullptr, slots };
+	a.inel = { base + NA_OFF_INEL, nullptr, slots };
+	a.spill = { base + NA_OFF_SPILL, nullptr, NA_SPILL };
+	a.spill_cap = NA_SPILL;
+	a.ctrl = ctrl;
+	a.det = out;
+	a.det_sc = nullptr;
+	a.dep = nullptr;
+	a.dep_cap = 0;
+	a.map.np = 0;
+	std::memset(a.map.pidx, -1, sizeof(a.map.pidx));
+	a.det_cap = out_cap;
+	a.slots = slots;
+	a.K = steps_per_launch;
+	const uint32_t g_refill = blocks_for(capacity);
+	const uint32_t g_step = std::min<uint32_t>(blocks_for(capacity), 64u * (uint32_t)dev->n_cu);
+	hipLaunchKernelGGL(k_na_init, dim3(1), dim3(64), 0, st, ctrl, capacity, (uint64_t)n);
+	Ctrl h;
+	uint32_t frame = 4;
+	uint64_t last_steps = ~0ull;
+	uint32_t last_det = 0;
+	int stalled = 0;
+	for (;;) {
+		for (uint32_t it = 0; it < frame; ++it) {
+			hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(64), 0, st, ctrl, slots, NA_SPILL);
+			hipLaunchKernelGGL(k_refill, dim3(g_refill), dim3(256), 0, st, ctrl, a.buf[0], a.buf[1], a.spill,
+				primaries, tags, (const uint64_t*)nullptr, (const uint16_t*)nullptr, 0u, sid_base);
+			hipLaunchKernelGGL((k_step<false, false>), dim3(g_step), dim3(256), 0, st, a);
+			hipLaunchKernelGGL((k_inel<false, false>), dim3(g_step), dim3(256), 0, st, a);
+		}
+		HIP_OK(hipGetLastError());
+		HIP_OK(hipMemcpyAsync(&h, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
+		HIP_OK(hipStreamSynchronize(st));
+		if (steps)
+			*steps = h.steps;
+		if (h.det_ctr > out_cap) {
+			*n_detected = out_cap;
+			return fail(NBL_ERR_CAPACITY, "nbl_track_noalloc: more electrons detected than the output "
+				"array holds (" + std::to_string(out_cap) + ")");
+		}
+		*n_detected = h.det_ctr;
+		if (active_count(h, NA_SPILL) == 0 && h.queue_n == h.queue_next)
+			break;
+		// the spill stack cannot grow here: a run whose buffer and stack are
+		// full of particles waiting for secondary slots ends with an error
+		if (h.steps == last_steps && h.det_ctr == last_det) {
+			if (++stalled >= 4)
+				return fail(NBL_ERR_CAPACITY, "nbl_track_noalloc: no progress: particle buffer and spill "
+					"stack full of particles waiting for secondary-electron slots");
+		} else {
+			stalled = 0;
+		}
+		last_steps = h.steps;
+		last_det = h.det_ctr;
+		frame = std::min<uint32_t>(frame * 2, 32);
+	}
 	return NBL_OK;
 }
 
