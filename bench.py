@@ -60,6 +60,48 @@ HBM_PEAK_GBS = 8000.0
 # primary is read once and each detected electron written once (36 B).
 BYTES_PER_STEP = 74
 BYTES_PER_RECORD = 36
+# 36-byte .pri / output record: position, direction, energy, pixel
+REC36 = np.dtype([("pos", "<f4", 3), ("dir", "<f4", 3), ("E", "<f4"), ("px", "<i4"), ("py", "<i4")])
+
+
+def yield_check(dev, geom, gm, sample, ref_records):
+    """The GPU on the reference's crop against nebula_cpu_mt's records of it:
+    BSE (E >= 50 eV) and SE (E < 50 eV) yields compared by z-score with
+    per-primary variances (every record names its primary through its pixel
+    label), and the exit-energy histograms (1 eV bins below 50 eV, 10 eV
+    above) by a two-sample chi-square.  No shared random stream: agreement
+    is statistical."""
+    n = len(sample)
+    d = A.Driver(dev, 1 << 21, geom, gm, seed=11, steps_per_launch=5)
+    d.push(sample, np.arange(n, dtype=np.uint32))
+    det = d.run(50 * n)
+    d.close()
+    own_ref = ref_records["px"].astype(np.int64) + 4096 * ref_records["py"].astype(np.int64)
+    e_ref, e_gpu, own_gpu = ref_records["E"], det["E"], det["tag"].astype(np.int64)
+
+    def per_primary(e, own):
+        return np.bincount(own[e >= 50], minlength=n), np.bincount(own[e < 50], minlength=n)
+
+    def z(a, b):
+        return float(abs(a.mean() - b.mean()) / np.sqrt(a.var(ddof=1) / n + b.var(ddof=1) / n + 1e-300))
+
+    b1, s1 = per_primary(e_ref, own_ref)
+    b2, s2 = per_primary(e_gpu, own_gpu)
+    bins = np.concatenate([np.arange(0, 50, 1.0), np.arange(50, float(sample["E"].max()) + 20, 10.0)])
+    h1, _ = np.histogram(e_ref, bins)
+    h2, _ = np.histogram(e_gpu, bins)
+    keep = (h1 + h2) > 0
+    h1, h2 = h1[keep].astype(float), h2[keep].astype(float)
+    k1, k2 = np.sqrt(h2.sum() / h1.sum()), np.sqrt(h1.sum() / h2.sum())
+    chi2 = float(np.sum((k1 * h1 - k2 * h2) ** 2 / (h1 + h2)))
+    dof = int(keep.sum()) - 1
+    zb, zs = z(b1, b2), z(s1, s2)
+    return {"sample": f"the cpu_baseline crop ({n} primaries), GPU seed 11",
+            "bse_reference": float(b1.mean()), "bse_gpu": float(b2.mean()), "z_bse": zb,
+            "se_reference": float(s1.mean()), "se_gpu": float(s2.mean()), "z_se": zs,
+            "exit_energy_chi2": chi2, "exit_energy_dof": dof,
+            "within_3sigma": bool(zb < 3 and zs < 3),
+            "chi2_within_5sd": bool(chi2 < dof + 5 * np.sqrt(2 * dof))}
 
 
 def workload(nx=256, ny=256, per_pixel=100, energy=300.0, seed=1):
@@ -72,7 +114,9 @@ def workload(nx=256, ny=256, per_pixel=100, energy=300.0, seed=1):
 
 
 def cpu_baseline(mesh, pri_full, mats, target_s=20.0, threads=None):
-    """Time the reference CPU binary (or the oracle) on a crop of the scan."""
+    """Time the reference CPU binary (or the oracle) on a crop of the scan.
+    Returns (the baseline entry, (crop, the binary's output records) or
+    None when the oracle was timed)."""
     tmp = tempfile.mkdtemp()
     try:
         return _cpu_baseline(mesh, pri_full, mats, target_s, threads, tmp)
@@ -80,15 +124,33 @@ def cpu_baseline(mesh, pri_full, mats, target_s=20.0, threads=None):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+def host_cores():
+    """Cores this process may use, and the share the CPU leg takes of them:
+    all of them, as the reference runs hardware_concurrency() threads
+    (cpu_mt_main.cpp:204), but at most 16, the host share of one GPU on a
+    shared GPU host."""
+    try:
+        visible = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        visible = os.cpu_count() or 1
+    return visible, min(16, visible)
+
+
 def _cpu_baseline(mesh, pri_full, mats, target_s, threads, tmp):
-    threads = threads or min(16, os.cpu_count() or 1)
+    visible, share = host_cores()
+    threads = threads or share
     ref_bin = os.path.join(ROOT, "oracle", "_ref", "nebula_cpu_mt")
-    # crop: a block of pixels from the scan's centre, all primaries of each
+    # crop: a block of pixels from the scan's centre, all primaries of each;
+    # each primary gets its own pixel label (i % 4096, i // 4096), which the
+    # physics never reads, so every output record names its primary
     ix, iy = pri_full["px"], pri_full["py"]
 
     def crop(side):
         sel = (np.abs(ix - 128) < side / 2) & (np.abs(iy - 128) < side / 2)
-        return pri_full[sel]
+        c = pri_full[sel].copy()
+        i = np.arange(len(c))
+        c["px"], c["py"] = i % 4096, i // 4096
+        return c
 
     if os.path.exists(ref_bin):
         try:
@@ -100,32 +162,37 @@ def _cpu_baseline(mesh, pri_full, mats, target_s, threads, tmp):
                 A.write_mat_file(p, m)
                 matfiles.append(p)
 
+            cpus = sorted(os.sched_getaffinity(0))[:threads] if hasattr(os, "sched_getaffinity") \
+                else list(range(threads))
+
             def run(sample):
                 pf = os.path.join(tmp, "s.pri")
                 W.write_pri(pf, sample)
-                cmd = ["taskset", "-c", f"0-{threads - 1}", ref_bin, tri, pf, *matfiles]
-                r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True,
-                                   timeout=600)
+                cmd = ["taskset", "-c", ",".join(map(str, cpus)), ref_bin, tri, pf, *matfiles]
+                r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=600)
+                err = r.stderr.decode(errors="replace")
                 if r.returncode != 0:
-                    raise RuntimeError(r.stderr[-500:])
-                m = re.search(r"Simulation: *([0-9.]+)", r.stderr)
-                return float(m.group(1))
+                    raise RuntimeError(err[-500:])
+                m = re.search(r"Simulation: *([0-9.]+)", err)
+                return float(m.group(1)), np.frombuffer(r.stdout, dtype=REC36)
 
             # the binary's Simulation clock advances in whole seconds (its
             # progress loop sleeps 1 s), so grow the crop until it runs >= 4 s
             side = 8
-            t = run(crop(side))
+            t, _ = run(crop(side))
             while t < 4.0 and side < 256:
                 side *= 2
-                t = run(crop(side))
+                t, _ = run(crop(side))
             # scale the crop to about target_s of work
             want = int(np.sqrt(side * side * target_s / max(t, 1e-3)))
             side = int(np.clip(want, side, 256))
             sample = crop(side)
-            t = run(sample)
+            t, records = run(sample)
             return {"value": len(sample) / t, "unit": "primaries/s", "cores": threads, "kind": "reference",
+                    "host_cores_visible": visible,
                     "sample": f"{side}x{side}-pixel crop of the scan ({len(sample)} primaries), "
-                              f"nebula_cpu_mt Simulation phase {t:.2f} s"}
+                              f"nebula_cpu_mt Simulation phase {t:.2f} s on {threads} threads"}, \
+                (sample, records)
         except Exception as e:  # noqa: BLE001
             print(f"reference CPU binary unusable here ({e}); timing the oracle", file=sys.stderr)
     sys.path.insert(0, ROOT)
@@ -142,7 +209,8 @@ def _cpu_baseline(mesh, pri_full, mats, target_s, threads, tmp):
             break
         side *= 2
     return {"value": len(sample) / t, "unit": "primaries/s", "cores": threads, "kind": "port",
-            "sample": f"{side}x{side}-pixel crop ({len(sample)} primaries), oracle Philox driver"}
+            "host_cores_visible": visible,
+            "sample": f"{side}x{side}-pixel crop ({len(sample)} primaries), oracle Philox driver"}, None
 
 
 def bench_batched(dev, scale=1.0):
@@ -412,9 +480,9 @@ def main():
         return
 
     mesh, pri, mats = workload(args.pixels, args.pixels, args.per_pixel)
-    cpu = None
+    cpu, ref_crop = None, None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(mesh, pri, mats)
+        cpu, ref_crop = cpu_baseline(mesh, pri, mats)
         print(f"cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     dev = A.Device(0)
@@ -447,15 +515,22 @@ def main():
         ms = ctypes.c_float()
         A.check(A.lib().nbl_device_copy_probe(dev._h, 1 << 30, ctypes.byref(ms)))
         print(f"copy probe: 1 GiB in {ms.value:.3f} ms", file=sys.stderr)
-    traffic = None
+    # measured HBM bytes per particle-step of the step pair (from the PMC
+    # runs of the same configuration), stated per launch of this run
+    traffic, traffic_per_step = None, None
     if os.path.exists(args.traffic_file):
         with open(args.traffic_file) as f:
             tr = json.load(f)
         same = all(tr.get("config", {}).get(k) == v for k, v in
                    (("pixels", args.pixels), ("per_pixel", args.per_pixel), ("capacity", args.capacity),
                     ("steps_per_launch", args.steps_per_launch)))
-        if same:
-            traffic = tr["bytes_per_launch"]
+        if same and "bytes_per_particle_step" in tr:
+            traffic_per_step = tr["bytes_per_particle_step"]
+            traffic = traffic_per_step * n_steps / max(launches, 1)
+    ycheck = None
+    if ref_crop is not None:
+        ycheck = yield_check(dev, geom, gm, *ref_crop)
+        print(f"yield check: {ycheck}", file=sys.stderr, flush=True)
     batched = None if args.no_batched else bench_batched(dev, args.batch_scale)
     grad = None if args.no_grad else bench_grad(dev)
     value = n_prim / t
@@ -483,10 +558,13 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_bytes_per_particle_step": traffic_per_step,
+                     "alg_bytes_per_particle_step": b_alg / max(n_steps, 1),
                      "kernel": "k_step+k_inel", "launches": launches,
                      "avg_launch_us": 1e3 * step_ms / max(launches, 1),
                      "alg_bytes_per_launch": b_alg / max(launches, 1)},
         "cpu_baseline": cpu,
+        "yield_check_vs_reference": ycheck,
         "particle_steps_per_primary": n_steps / n_prim,
         "batched": batched,
         "grad": grad,
