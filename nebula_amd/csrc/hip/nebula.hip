@@ -32,8 +32,8 @@ using namespace nbl::dev;
 	return fail(NBL_ERR_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); } while (0)
 
 // ------------------------------------------------------------- buffers --
-// One allocation per buffer of n entries: sid (8 bytes x n), then twelve
-// 4-byte fields x n (px py pz dx dy dz E mat last_tri tag step flags).  A
+// One allocation per buffer of n entries: sid (8 bytes x n), then thirteen
+// 4-byte fields x n (px py pz dx dy dz E mat last_tri tag step flags safe).  A
 // buffer is one base pointer and its size, so a kernel holds four buffers
 // in a few scalar registers instead of 13 pointers each.
 struct Bufs {
@@ -41,7 +41,7 @@ struct Bufs {
 	double* sc;             // scores, [parameter][entry], or null
 	uint32_t sc_stride;     // entries of this buffer (n)
 };
-enum { F_PX, F_PY, F_PZ, F_DX, F_DY, F_DZ, F_E, F_MAT, F_LAST, F_TAG, F_STEP, F_FLAGS, N_FIELDS };
+enum { F_PX, F_PY, F_PZ, F_DX, F_DY, F_DZ, F_E, F_MAT, F_LAST, F_TAG, F_STEP, F_FLAGS, F_SAFE, N_FIELDS };
 
 __device__ __forceinline__ float* fld_f(const Bufs& b, int f)
 {
@@ -124,6 +124,7 @@ __device__ __forceinline__ State load_state(const Bufs& b, uint32_t i)
 	s.tag = fld_u(b, F_TAG)[i];
 	s.step = fld_u(b, F_STEP)[i];
 	s.flags = fld_u(b, F_FLAGS)[i];
+	s.safe = fld_f(b, F_SAFE)[i];
 	s.sid = ((const uint64_t*)b.base)[i];
 	return s;
 }
@@ -138,6 +139,7 @@ __device__ __forceinline__ void store_state(const Bufs& b, uint32_t i, const Sta
 	fld_u(b, F_TAG)[i] = s.tag;
 	fld_u(b, F_STEP)[i] = s.step;
 	fld_u(b, F_FLAGS)[i] = s.flags;
+	fld_f(b, F_SAFE)[i] = s.safe;
 	((uint64_t*)b.base)[i] = s.sid;
 }
 
@@ -306,6 +308,7 @@ __global__ __launch_bounds__(256) void k_refill(Ctrl* c, Bufs b0, Bufs b1, Bufs 
 	s.step = 0;
 	s.flags = qworld ? ((uint32_t)qworld[src] << FL_WORLD_SHIFT) : 0u;
 	s.sid = qsid ? qsid[src] : sid_base + src;
+	s.safe = -1.0f;
 	store_state(b, dst, s);
 	if (b.sc)
 		for (uint32_t q = 0; q < np; ++q)
@@ -340,6 +343,8 @@ __device__ __forceinline__ bool place_secondary(const StepArgs& a, Ctrl* C, cons
 		ns.step = 0;
 		ns.flags = parent.flags & FL_WORLD_MASK;
 		ns.sid = se_sid(parent.sid, parent.step);
+		// it starts where its parent stands, so the parent's safety holds
+		ns.safe = parent.safe;
 		const bool spill = sslot >= se_limit;
 		const uint32_t pslot = wave_alloc(&C->spill_ctr, spill);
 		if (!spill) {
@@ -408,7 +413,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	World Wl = a.W;
 	const World& W = MULTI ? Wl : a.W;
 	__shared__ int s_path[PATH_LEVELS * 256];
-	__shared__ int s_owner[3 * 256];   // per wave: owner table, pair queue
+	__shared__ int s_owner[4 * 256];   // per wave: owner table, pair queue, safeties
 	// In the plain variant a particle's stream id waits here during its steps
 	// (each step reads it once to start its Philox stream): two fewer VGPRs
 	// across the step loop, and the loop no longer spills (4 % faster).  The
@@ -416,7 +421,18 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	constexpr bool SID_LDS = !SC && !MULTI;
 	__shared__ uint64_t s_sid[SID_LDS ? 256 : 1];
 	int* path = s_path + threadIdx.x;
-	int* owner_at = s_owner + 3 * (threadIdx.x & ~63u);
+	int* owner_at = s_owner + 4 * (threadIdx.x & ~63u);
+	// Steps whose free path cannot reach a triangle skip the walk: after a
+	// walk that found no hit, `rem` is the start point's safety (no triangle
+	// of its cell and no cell wall within that distance, propagate_wave)
+	// less the distance moved since; a later free path no longer than `rem`
+	// ends inside that ball, where the walk would find no hit and return the
+	// full free path.  rem < 0: the next step walks.  It travels with the
+	// particle (State::safe) from launch to launch, through the inelastic
+	// queue (the event does not move it) and into its secondaries (they
+	// start where it stands).  (Single-world variants; the multi-world walk
+	// gives no safety and clears it.)
+	constexpr bool SKIP = !MULTI;
 	uint32_t my_steps = 0;
 	const uint32_t stride = gridDim.x * blockDim.x;
 	// Whole waves take consecutive particles and step in lockstep: the walk
@@ -442,6 +458,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 		if (SID_LDS)
 			s_sid[threadIdx.x] = s.sid;
 		bool run = valid, det = false, to_inel = false, dead = false;
+		float rem = SKIP ? s.safe : -1.0f;
 		for (uint32_t k = 0; k < K; ++k) {
 			if (__ballot(run) == 0ull)
 				break;
@@ -467,9 +484,17 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 					walk = (oc == ALIVE);
 				}
 			}
-			float hd;
-			const int hit = propagate_wave<!MULTI>(W.g, walk, s.p.pos, nd, distance, s.last_tri, s.mat, hd,
-				path, 256, owner_at);
+			const bool skip = SKIP && walk && distance <= rem;
+			float hd, saf = -1.0f;
+			int hit = propagate_wave<!MULTI, SKIP>(W.g, walk && !skip, s.p.pos, nd, distance, s.last_tri, s.mat,
+				hd, path, 256, owner_at, &saf);
+			if (SKIP && walk) {
+				if (skip) {
+					hit = -1;
+					hd = distance;
+				}
+				rem = (hit >= 0) ? -1.0f : ((skip ? rem : saf) - distance) - W.g.step_eps;
+			}
 			if (walk) {
 				int ev;
 				step_move(s, nd, distance, type, hit, hd, ev, pinfo);
@@ -509,6 +534,8 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 		}
 		if (SID_LDS)
 			s.sid = s_sid[threadIdx.x];
+		// the multi-world kernel moves particles without a safety: clear it
+		s.safe = rem;
 		bool keep = valid && !dead;
 		const uint32_t dslot = wave_alloc(&C->det_ctr, det);
 		if (det) {
@@ -663,7 +690,7 @@ __global__ __launch_bounds__(256) void k_query(Geometry g, const float* pos, con
 	float* out_dist, uint32_t n)
 {
 	__shared__ int s_path[PATH_LEVELS * 256];
-	__shared__ int s_owner[3 * 256];   // per wave: owner table, pair queue
+	__shared__ int s_owner[4 * 256];   // per wave: owner table, pair queue, safeties
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	const bool valid = i < n;
 	F3 p = { 0.0f, 0.0f, 0.0f }, d = { 1.0f, 0.0f, 0.0f };
@@ -679,7 +706,7 @@ __global__ __launch_bounds__(256) void k_query(Geometry g, const float* pos, con
 	// the walk of the step kernel: whole waves, candidate tests shared
 	float od;
 	const int r = propagate_wave<true>(g, valid, p, d, dd, it, im, od, s_path + threadIdx.x, 256,
-		s_owner + 3 * (threadIdx.x & ~63u));
+		s_owner + 4 * (threadIdx.x & ~63u));
 	if (valid) {
 		out_tri[i] = r;
 		out_dist[i] = od;
@@ -727,6 +754,7 @@ __global__ __launch_bounds__(256) void k_step_states(World W, nbl_step_state* st
 	s.p.E = h.kin_energy;
 	s.mat = h.mat; s.last_tri = h.last_tri; s.tag = h.tag; s.step = h.step; s.flags = h.flags;
 	s.sid = h.sid;
+	s.safe = -1.0f;
 	Rng rng;
 	rng_start(rng, W.key0, W.key1, s.sid, s.step);
 	int ev = EV_NONE, hit = -1;
@@ -1138,6 +1166,8 @@ int nbl_geometry_upload(nbl_device* dev, const int32_t* od, int64_t n_od, const 
 	const float h2 = halfsize[0] * halfsize[0] + halfsize[1] * halfsize[1] + halfsize[2] * halfsize[2];
 	g->g.max_extent = 2 * std::sqrt(h2);
 	g->g.ntri = (int)ntri;
+	g->g.step_eps = 3e-7f * (std::fabs(center[0]) + halfsize[0] + std::fabs(center[1]) + halfsize[1] +
+		std::fabs(center[2]) + halfsize[2]) + 1e-6f;
 	g->max_material = -1;
 	for (int64_t i = 0; i < ntri; ++i)
 		g->max_material = std::max(g->max_material, std::max(tris[i].material_in, tris[i].material_out));

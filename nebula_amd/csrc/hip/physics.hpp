@@ -474,6 +474,10 @@ struct Geometry {
 	F3 center, halfsize;
 	float max_extent;
 	int ntri;
+	// Bound on the rounding of one move inside the domain (a position
+	// update's error, summed over components, plus the safety's own
+	// subtraction): 3e-7 x the domain's largest 1-norm + 1e-6 nm.
+	float step_eps;
 };
 
 // core/triangle.inl:7-28
@@ -652,6 +656,53 @@ __device__ __forceinline__ bool cell_may_hit(const Geometry& g, int entry, F3 cu
 	return !((fa > tol && fb > tol) || (fa < -tol && fb < -tol));
 }
 
+// Safety of a walk's start point: a radius S such that any later ray of
+// length d starting within distance r of `cur`, with r + d <= S, finds no
+// hit in the walk (the walk returns "no hit" and the full free path), so
+// that such a step can skip the walk with the same result.
+//
+// Triangle part, for one triangle of the start cell from its screening
+// record: with s = cur - r0, n the exact e1 x e2 and t the computed
+// intersect_ray distance of a ray from p' = cur + D (|D| <= r) along any
+// unit direction, intersect_ray's numerator and determinant are the exact
+// s'.n and -dir.n up to rounding errors of at most 1e-6 |s'|_1 |e1|_1 |e2|_1
+// and 1e-6 |e1|_1 |e2|_1 (a few unit roundoffs of each product, with
+// |dir|_1 <= sqrt(3)), and the record's n is within 2e-7 |e1|_1 |e2|_1 of
+// the exact one.  Hence |t| >= (|s.n| - r |n| - err) / (|n| + err'), and
+// |t| >= d holds whenever a - 2e-6 |s|_1 ee >= S (|n|~ (1 + 2e-6) + 3e-6 ee)
+// with a = |s.n| as computed here, ee = |e1|_1 |e2|_1 and |n|~ >= |n| the
+// record's norm rounded up (the terms in r double the error allowance,
+// r <= S).  A computed t >= d (= the remaining free path) is never taken as
+// a hit: the walk keeps the nearest t <= (cell exit + EPS) and reports a hit
+// only below the free path.  The bound below keeps a further 1e-4 relative
+// slack for its own rounding (reciprocal and square root within 1 ulp).
+// Triangles skipped by the material or last-triangle tests count too: the
+// direction and the last triangle change between steps.
+__device__ __forceinline__ float pair_safety(const Geometry& g, int entry, F3 cur)
+{
+	const float4 a = g.scr[2 * entry], b = g.scr[2 * entry + 1];
+	const F3 nrm = { b.x, b.y, b.z };
+	const F3 sv = { cur.x - a.x, cur.y - a.y, cur.z - a.z };
+	const float ee = a.w;
+	const float num = fabsf(dot(sv, nrm)) - 2e-6f * norm1(sv) * ee;
+	const float den = sqrtf(dot(nrm, nrm)) * 1.0001f + 1e-5f * ee;
+	return num * __builtin_amdgcn_rcpf(den) * 0.9999f;
+}
+
+// Wall part: distance from `cur` to the walls of its leaf cell (centre
+// a, half extents aw x halfsize), less a margin that covers the rounding of
+// cell_exit's exit distances (a few unit roundoffs of the coordinates'
+// magnitudes, and EPS pushes the walls outwards): a segment that ends at
+// least that far inside the cell is seen to end before the cell's exit.
+__device__ __forceinline__ float wall_safety(const Geometry& g, F3 cur, float ax, float ay, float az, float aw)
+{
+	const F3 hs = g.halfsize;
+	const float hx = aw * hs.x, hy = aw * hs.y, hz = aw * hs.z;
+	const float w = fminf(fminf(hx - fabsf(cur.x - ax), hy - fabsf(cur.y - ay)), hz - fabsf(cur.z - az));
+	const float mag = fabsf(ax) + fabsf(ay) + fabsf(az) + hx + hy + hz;
+	return w - (1e-3f + 4e-6f * mag);
+}
+
 // The neighbour across wall `target` (-1/-2/-4): flip that axis' bit at the
 // deepest level where the move stays inside the parent; location 1 (or 0)
 // means the ray left the octree.
@@ -784,11 +835,19 @@ __device__ __forceinline__ int chunk_owner(int base, int excl, int n, int* owner
 //
 // SHARED: every lane walks the same geometry.  Otherwise (lanes of different
 // scans) the candidate tests read the owning lane's triangle tables.
-template <bool SHARED>
+//
+// SAF (with SHARED): also return in `saf` the safety of each walking lane's
+// start point (wall_safety of its first cell, and pair_safety of every
+// triangle listed there, minimised per owner in the 64 ints of LDS after
+// the owner table and queue), or -1 for a lane that does not walk.
+template <bool SHARED, bool SAF = false>
 __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 dir, float distance,
-	int ignore_tri, int ignore_mat, float& out_dist, int* __restrict__ path, int stride, int* owner_at)
+	int ignore_tri, int ignore_mat, float& out_dist, int* __restrict__ path, int stride, int* owner_at,
+	float* saf = nullptr)
 {
 	out_dist = distance;
+	if (SAF)
+		*saf = -1.0f;
 	if (__ballot(1) != ~0ull) {
 		int r = -1;
 		if (walk)
@@ -802,6 +861,8 @@ __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 
 	uint32_t cells = 0;
 	int result = -1;
 	bool active = walk;
+	bool first = true;   // the walk's first round: every walking lane in its start cell
+	unsigned* s_saf = (unsigned*)owner_at + 192;
 	while (__ballot(active) != 0ull) {
 		int n = 0, off = 0, target = 0;
 		float isect = 0.0f;
@@ -810,6 +871,8 @@ __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 
 			off = cell_find(g, cur, location, ax, ay, az, aw, path, stride, n);
 			++cells;
 			cell_exit(g, cur, dir, ax, ay, az, aw, isect, target);
+			if (SAF && first)
+				s_saf[lane] = __float_as_uint(fmaxf(wall_safety(g, cur, ax, ay, az, aw), 0.0f));
 		}
 		const int incl = wave_incl_sum(n);
 		const int excl = incl - n;
@@ -860,6 +923,8 @@ __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 
 					ti = g.od[ooff + p];
 					const bool may = cell_may_hit(g, ooff + p, oc, odir, oign_mat, oto);
 					pass = may && ti != oign_tri;
+					if (SAF && first)
+						atomicMin(s_saf + o, __float_as_uint(fmaxf(pair_safety(g, ooff + p, oc), 0.0f)));
 				}
 				const uint64_t pm = __ballot(pass);
 				const int np = (int)__popcll(pm);
@@ -876,6 +941,12 @@ __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 
 			}
 			if (qn > 0)
 				flush();
+			if (SAF && first) {
+				__builtin_amdgcn_wave_barrier();
+				if (walk)
+					*saf = __uint_as_float(s_saf[lane]);
+				__builtin_amdgcn_wave_barrier();
+			}
 		} else {
 			// Lanes of different worlds (the multi-world kernel, 4 waves per
 			// SIMD): every pair is tested in full; the screen's queue and
@@ -921,6 +992,7 @@ __device__ inline int propagate_wave(const Geometry& g, bool walk, F3 start, F3 
 				}
 			}
 		}
+		first = false;
 		if (active) {
 			if (isect >= to_go) {
 				out_dist = distance;
@@ -1255,6 +1327,10 @@ struct State {
 	int mat, last_tri;
 	uint32_t tag, step, flags;
 	uint64_t sid;
+	// Safety left of the particle's last walk: no triangle of that walk's
+	// start cell and no wall of that cell lies within this distance of the
+	// particle (the step kernel's walk skip); < 0 when unknown.
+	float safe;
 };
 
 struct World {
