@@ -298,6 +298,19 @@ int nbl_driver_steps(nbl_driver* d, uint64_t* steps);
  * more than cap electrons were detected. */
 int nbl_driver_run(nbl_driver* d, nbl_detected* out, uint64_t cap, uint64_t* n_detected);
 
+/* The frame loop of gpu_main.cu:200-230 with buffer_detected/flush_buffered
+ * (drivers/gpu/gpu_driver.inl:255-307): run until every pushed primary and
+ * its cascade has finished, frame_iters iterations per frame.  After each
+ * frame its detected electrons are copied on the stream into pinned memory
+ * and the next frame is enqueued at once; then sink(user, records, n,
+ * running, queued) receives them (running / queued: particles in flight
+ * and primaries not yet started, for progress output) while the GPU runs
+ * that frame.  The records are valid until the sink returns; a nonzero
+ * return stops the run (NBL_ERR_ARG). */
+typedef int (*nbl_record_sink)(void* user, const nbl_detected* records, uint32_t n, uint32_t running,
+	uint64_t queued);
+int nbl_driver_run_frames(nbl_driver* d, uint32_t frame_iters, nbl_record_sink sink, void* user);
+
 /* The HIP stream the driver launches on (a hipStream_t). */
 void* nbl_driver_stream(nbl_driver* d);
 

@@ -160,6 +160,7 @@ _SIGS = {
     "nbl_driver_steps": (C.c_int, [C.c_void_p, _p(C.c_uint64)]),
     "nbl_driver_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, _p(C.c_uint64)]),
     "nbl_driver_stream": (C.c_void_p, [C.c_void_p]),
+    "nbl_driver_run_frames": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "nbl_track_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32, C.c_float,
                                     C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
                                     C.c_uint64, C.c_void_p, C.c_uint32, _p(C.c_uint32), _p(C.c_uint64)]),

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import weakref
 
 import numpy as np
 
@@ -127,6 +128,12 @@ class Device:
         self._h = C.c_void_p()
         check(lib().nbl_device_open(ordinal, C.byref(self._h)))
         self.ordinal = ordinal
+        # objects made on this device, closed before it (drivers first: they
+        # use geometries and materials)
+        self._drivers, self._others = weakref.WeakSet(), weakref.WeakSet()
+
+    def _register(self, obj, driver: bool = False):
+        (self._drivers if driver else self._others).add(obj)
 
     def memory(self):
         """(free, total) device memory in bytes."""
@@ -139,6 +146,9 @@ class Device:
 
     def close(self):
         if self._h:
+            for group in (self._drivers, self._others):
+                for obj in list(group):
+                    obj.close()
             lib().nbl_device_close(self._h)
             self._h = C.c_void_p()
 
@@ -163,6 +173,7 @@ class Geometry:
                                         self.center.ctypes.data_as(C.POINTER(C.c_float)),
                                         self.halfsize.ctypes.data_as(C.POINTER(C.c_float)),
                                         C.byref(self._h)))
+        device._register(self)
 
     @property
     def aabb_min(self):
@@ -314,6 +325,7 @@ class Material:
         self.device = device
         self._h = C.c_void_p()
         check(lib().nbl_material_upload(device._h, host._h, C.byref(self._h)))
+        device._register(self)
 
     def probe(self, which: int, mode: int, x, y=None, z=None):
         x = np.ascontiguousarray(x, np.float32)
@@ -382,6 +394,10 @@ def to_particles(pri: np.ndarray) -> np.ndarray:
     return p
 
 
+# nbl_record_sink: int (void* user, const nbl_detected*, uint32_t n, uint32_t running, uint64_t queued)
+RECORD_SINK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint64)
+
+
 class Driver:
     """gpu_driver<scatter_physics<true>, intersect_t, octree<true>>
     (drivers/gpu/gpu_driver.h:56-178), run by libnebula.so."""
@@ -396,6 +412,7 @@ class Driver:
         check(lib().nbl_driver_create(device._h, capacity, geometry._h, arr, len(self.materials),
                                       physics, energy_threshold, seed, steps_per_launch,
                                       C.byref(self._h)))
+        device._register(self, driver=True)
         self.seed = seed
         # launch configuration (the library's defaults until set_launch / autotune)
         self.launch = {"steps_per_launch": steps_per_launch, "blocks_per_cu": 64, "target": capacity}
@@ -647,6 +664,27 @@ class Driver:
         made = np.zeros(n, np.int32)
         check(lib().nbl_driver_step_states(self._h, ptr(st), ptr(ev), ptr(oc), ptr(se), ptr(made), n))
         return st, ev, oc, se, made
+
+    def run_frames(self, frame_iters: int, on_records) -> None:
+        """nbl_driver_run_frames: track everything pushed, frame_iters
+        iterations per frame; on_records(records, running, queued) receives
+        each frame's detected electrons (a structured array copy) while the
+        GPU runs the next frame.  An exception in on_records stops the run."""
+        err = []
+
+        def sink(user, recs, n, running, queued):
+            try:
+                raw = C.string_at(recs, n * _lib.DET_DTYPE.itemsize) if n else b""
+                on_records(np.frombuffer(raw, dtype=_lib.DET_DTYPE).copy(), running, queued)
+                return 0
+            except Exception as e:  # noqa: BLE001 - handed back after the run
+                err.append(e)
+                return 1
+        cb = RECORD_SINK(sink)
+        rc = lib().nbl_driver_run_frames(self._h, frame_iters, C.cast(cb, C.c_void_p), None)
+        if err:
+            raise err[0]
+        check(rc)
 
     def close(self):
         if self._h:

@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -80,7 +81,9 @@ struct Ctrl {
 	// and the two a wave always needs together share one 64-bit word, so
 	// that one atomic returns both bases.
 	alignas(128) uint64_t step_out;   // k_step: survivors (low 32) | inelastic queue (high 32)
-	alignas(128) uint32_t se_ctr;     // k_inel: secondaries made (one atomic per block)
+	// k_inel, one atomic per block: secondaries made (low 32) | secondaries
+	// dropped at birth, whose one step k_ctrl adds to `steps` (high 32)
+	alignas(128) uint64_t se_ctr;
 	alignas(128) uint32_t det_ctr;    // detected records written (may overshoot det_cap)
 	alignas(128) uint32_t spill_ctr;  // spill stack top during a k_step (may overshoot)
 	alignas(128) uint32_t dep_ctr;    // energy-deposit records written (may overshoot dep_cap)
@@ -93,7 +96,9 @@ __host__ __device__ inline uint32_t hi32(uint64_t v) { return (uint32_t)(v >> 32
 // survivors written by the last k_step + k_inel pair (k_inel keeps every
 // queued particle), secondaries requested
 __host__ __device__ inline uint32_t out_surv(const Ctrl& c) { return lo32(c.step_out) + hi32(c.step_out); }
-__host__ __device__ inline uint32_t out_se(const Ctrl& c) { return c.se_ctr; }
+__host__ __device__ inline uint32_t out_se(const Ctrl& c) { return lo32(c.se_ctr); }
+// particle-steps so far, with those of the last launch's dropped secondaries
+__host__ __device__ inline uint64_t total_steps(const Ctrl& c) { return c.steps + hi32(c.se_ctr); }
 
 struct StepArgs {
 	World W;
@@ -246,6 +251,7 @@ __global__ void k_ctrl(Ctrl* c, uint32_t slots, uint32_t spill_cap)
 	if (threadIdx.x != 0 || blockIdx.x != 0)
 		return;
 	if (c->stepped) {
+		c->steps += hi32(c->se_ctr);
 		c->buf_in ^= 1u;
 		c->n_surv = out_surv(*c);
 		c->n_se = min(out_se(*c), c->se_limit);
@@ -420,6 +426,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	// multi-world variant has the registers and ran 3 % slower with it.
 	constexpr bool SID_LDS = !SC && !MULTI;
 	__shared__ uint64_t s_sid[SID_LDS ? 256 : 1];
+
 	int* path = s_path + threadIdx.x;
 	int* owner_at = s_owner + 4 * (threadIdx.x & ~63u);
 	// Steps whose free path cannot reach a triangle skip the walk: after a
@@ -616,7 +623,7 @@ __global__ __launch_bounds__(256) void k_inel(StepArgs a)
 	World Wl = a.W;
 	const World& W = MULTI ? Wl : a.W;
 	const uint32_t stride = gridDim.x * blockDim.x;
-	__shared__ uint32_t s_cnt[4];
+	__shared__ uint32_t s_cnt[4], s_doa[4];
 	__shared__ uint32_t s_base;
 	const uint32_t wave = threadIdx.x >> 6;
 	for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += stride) {
@@ -625,11 +632,12 @@ __global__ __launch_bounds__(256) void k_inel(StepArgs a)
 		State s = {}, pre = {};
 		Score sc;
 		Particle se = {};
-		bool made = false;
+		bool made = false, doa = false;
 		if (valid) {
 			s = load_state(a.inel, i);
 			if (MULTI) {
 				const WorldDesc& wd = a.worlds[s.flags >> FL_WORLD_SHIFT];
+				Wl.g = wd.g;
 				Wl.mats = wd.mats;
 				Wl.nmats = wd.nmats;
 			}
@@ -639,14 +647,28 @@ __global__ __launch_bounds__(256) void k_inel(StepArgs a)
 			rng_start(rng, W.key0, W.key1, s.sid, s.step);
 			rng_seek(rng, 2);
 			step_event(W, s, rng, EV_INELASTIC, -1, se, made);
+			// A secondary that its first step would end at once (outside the
+			// domain, or below its material's barrier plus the threshold:
+			// step_plan's checks, gpu_driver.inl:423-433) is not stored: it
+			// leaves nothing behind, and its one step is counted (high half
+			// of se_ctr; its parent's event always completes).
+			if (made && (!in_domain(W.g, se.pos) || se.E < W.mats[s.mat].barrier + W.threshold)) {
+				made = false;
+				doa = true;
+			}
 		}
 		const uint64_t m = __ballot(made);
-		if (__lane_id() == 0)
+		const uint64_t md = __ballot(doa);
+		if (__lane_id() == 0) {
 			s_cnt[wave] = (uint32_t)__popcll(m);
+			s_doa[wave] = (uint32_t)__popcll(md);
+		}
 		__syncthreads();
 		if (threadIdx.x == 0) {
 			const uint32_t tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-			s_base = tot ? atomicAdd(&C->se_ctr, tot) : 0u;
+			const uint32_t totd = s_doa[0] + s_doa[1] + s_doa[2] + s_doa[3];
+			s_base = (tot | totd) ? lo32(atomicAdd((unsigned long long*)&C->se_ctr,
+				((unsigned long long)totd << 32) | tot)) : 0u;
 		}
 		__syncthreads();
 		uint32_t sslot = s_base + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
@@ -1624,6 +1646,8 @@ int nbl_driver_iterate(nbl_driver* d, uint32_t n_iter)
 		return fail(NBL_ERR_ARG, "null driver");
 	int rc = set_device(d->dev);
 	if (rc) return rc;
+	// an error left by an earlier, unrelated runtime call is not this call's
+	(void)hipGetLastError();
 	const StepArgs a = step_args(d);
 	if (d->use_graph && !d->timing && n_iter > 1) {
 		if ((rc = launch_graph(d, a, n_iter)))
@@ -1676,12 +1700,12 @@ int nbl_driver_autotune(nbl_driver* d, uint32_t iters_per_trial, nbl_tune_result
 		int r = nbl_driver_set_launch(d, c.K, c.bpc, c.target);
 		if (r) return r;
 		if ((r = read_ctrl(d))) return r;
-		const uint64_t s0 = d->h_ctrl->steps;
+		const uint64_t s0 = total_steps(*d->h_ctrl);
 		const double ms0 = d->step_ms;
 		if ((r = nbl_driver_iterate(d, iters_per_trial))) return r;
 		if ((r = read_ctrl(d))) return r;
 		const double dt = (d->step_ms - ms0) * 1e-3;
-		rate = dt > 0 ? (double)(d->h_ctrl->steps - s0) / dt : 0.0;
+		rate = dt > 0 ? (double)(total_steps(*d->h_ctrl) - s0) / dt : 0.0;
 		++trials;
 		// stop before the detected buffer or the queue runs out
 		const uint64_t left = d->h_ctrl->queue_n - d->h_ctrl->queue_next;
@@ -1955,18 +1979,21 @@ int nbl_driver_steps(nbl_driver* d, uint64_t* steps)
 	if (rc) return rc;
 	if ((rc = read_ctrl(d)))
 		return rc;
-	*steps = d->h_ctrl->steps;
+	*steps = total_steps(*d->h_ctrl);
 	return NBL_OK;
 }
 
 namespace {
-// Track everything pushed to the end.  Each frame's detected records are
-// copied on the stream into pinned staging and the counter reset, and the
-// next frame is enqueued at once; the host then moves the staged records
-// into `out` while the GPU runs that frame.
-int run(nbl_driver* d, nbl_detected* out, double* scores, uint64_t cap, uint64_t* n_detected)
+// The frame loop shared by nbl_driver_run and nbl_driver_run_frames: track
+// everything pushed to the end.  Each frame's detected records (and
+// scores) are copied on the stream into pinned staging and the record
+// counter reset, the next frame is enqueued at once, and the host hands the
+// staged records to `sink` while the GPU runs that frame.  Frames are
+// `frame_iters` iterations, or (0) 4 growing to 32.
+using RecordSink = std::function<int(const nbl_detected*, const double*, uint32_t, uint32_t, uint64_t)>;
+int run_loop(nbl_driver* d, bool with_scores, uint32_t frame_iters, const RecordSink& sink)
 {
-	const size_t np = scores ? (size_t)d->map.np : 0;
+	const size_t np = with_scores ? (size_t)d->map.np : 0;
 	if (!d->h_stage &&
 		hipHostMalloc((void**)&d->h_stage, sizeof(nbl_detected) * d->det_cap, hipHostMallocDefault) != hipSuccess) {
 		d->h_stage = nullptr;
@@ -1984,8 +2011,7 @@ int run(nbl_driver* d, nbl_detected* out, double* scores, uint64_t cap, uint64_t
 	}
 	if (!d->stage_ev)
 		HIP_OK(hipEventCreateWithFlags(&d->stage_ev, hipEventDisableTiming));
-	uint64_t total = 0;
-	uint32_t frame = 4;
+	uint32_t frame = frame_iters ? frame_iters : 4;
 	uint64_t last_steps = ~0ull;
 	int stalled = 0;
 	int rc = nbl_driver_iterate(d, frame);
@@ -2010,33 +2036,48 @@ int run(nbl_driver* d, nbl_detected* out, double* scores, uint64_t cap, uint64_t
 			// Every particle waiting for a secondary slot with the buffer and
 			// the spill stack full would never finish: report it instead of
 			// spinning.
-			if (d->h_ctrl->steps == last_steps && det == 0) {
+			if (total_steps(*d->h_ctrl) == last_steps && det == 0) {
 				if (++stalled >= 4)
 					return fail(NBL_ERR_CAPACITY, "no progress: particle buffer and spill stack full of "
 						"particles waiting for secondary-electron slots; raise the capacity");
 			} else {
 				stalled = 0;
 			}
-			last_steps = d->h_ctrl->steps;
-			frame = std::min<uint32_t>(frame * 2, 32);
+			last_steps = total_steps(*d->h_ctrl);
+			if (!frame_iters)
+				frame = std::min<uint32_t>(frame * 2, 32);
 			if ((rc = nbl_driver_iterate(d, frame)))
 				return rc;
 		}
-		if (det) {
+		if (det)
 			HIP_OK(hipEventSynchronize(d->stage_ev));
-			const uint64_t room = cap > total ? std::min<uint64_t>(cap - total, det) : 0;
-			if (room) {
-				std::memcpy(out + total, d->h_stage, sizeof(nbl_detected) * room);
-				if (np)
-					std::memcpy(scores + total * np, d->h_stage_sc, sizeof(double) * np * room);
-			}
-			total += det;
-		}
+		if ((rc = sink(d->h_stage, np ? d->h_stage_sc : nullptr, det, running, queued)))
+			return rc;
 		if (finished)
 			break;
 	}
 	HIP_OK(hipStreamSynchronize(d->stream));
+	return NBL_OK;
+}
+
+int run(nbl_driver* d, nbl_detected* out, double* scores, uint64_t cap, uint64_t* n_detected)
+{
+	const size_t np = scores ? (size_t)d->map.np : 0;
+	uint64_t total = 0;
+	const int rc = run_loop(d, scores != nullptr, 0,
+		[&](const nbl_detected* recs, const double* sc, uint32_t n, uint32_t, uint64_t) -> int {
+			const uint64_t room = cap > total ? std::min<uint64_t>(cap - total, n) : 0;
+			if (room) {
+				std::memcpy(out + total, recs, sizeof(nbl_detected) * room);
+				if (np)
+					std::memcpy(scores + total * np, sc, sizeof(double) * np * room);
+			}
+			total += n;
+			return NBL_OK;
+		});
 	*n_detected = total;
+	if (rc)
+		return rc;
 	if (total > cap)
 		return fail(NBL_ERR_CAPACITY, "more detected electrons than the output array holds");
 	return NBL_OK;
@@ -2060,6 +2101,21 @@ int nbl_driver_run_scores(nbl_driver* d, nbl_detected* out, double* scores, uint
 	if (!d->map.np)
 		return fail(NBL_ERR_ARG, "scores are not enabled on this driver");
 	return run(d, out, scores, cap, n_detected);
+}
+
+int nbl_driver_run_frames(nbl_driver* d, uint32_t frame_iters, nbl_record_sink sink, void* user)
+{
+	clear_error();
+	if (!d || !sink || frame_iters < 1 || frame_iters > 1u << 16)
+		return fail(NBL_ERR_ARG, "invalid run_frames arguments (frame_iters in [1, 65536], a sink)");
+	int rc = set_device(d->dev);
+	if (rc) return rc;
+	return run_loop(d, false, frame_iters,
+		[&](const nbl_detected* recs, const double*, uint32_t n, uint32_t running, uint64_t queued) -> int {
+			if (sink(user, recs, n, running, queued) != 0)
+				return fail(NBL_ERR_ARG, "the record sink stopped the run");
+			return NBL_OK;
+		});
 }
 
 int nbl_driver_set_material_params(nbl_driver* d, int32_t material, float el_mfp_scale,
@@ -2194,6 +2250,7 @@ int nbl_geometry_query(const nbl_geometry* g, const float* pos, const float* dir
 	HIP_OK(hipMemcpyAsync(ddist, dist, 4 * (size_t)n, hipMemcpyHostToDevice, s));
 	HIP_OK(hipMemcpyAsync(dit, ignore_tri, 4 * (size_t)n, hipMemcpyHostToDevice, s));
 	HIP_OK(hipMemcpyAsync(dim, ignore_mat, 4 * (size_t)n, hipMemcpyHostToDevice, s));
+	(void)hipGetLastError();   // not this call's
 	hipLaunchKernelGGL(k_query, dim3(blocks_for(n)), dim3(256), 0, s, g->g, dp, dd, ddist, dit, dim,
 		dtri, dout, n);
 	HIP_OK(hipGetLastError());
@@ -2232,6 +2289,7 @@ int nbl_table_probe(const nbl_material* m, int32_t which, int32_t mode, const fl
 	HIP_OK(hipMemcpyAsync(dx, x, 4 * (size_t)n, hipMemcpyHostToDevice, s));
 	HIP_OK(hipMemcpyAsync(dy, y ? y : x, 4 * (size_t)n, hipMemcpyHostToDevice, s));
 	HIP_OK(hipMemcpyAsync(dz, z ? z : x, 4 * (size_t)n, hipMemcpyHostToDevice, s));
+	(void)hipGetLastError();   // not this call's
 	hipLaunchKernelGGL(k_probe, dim3(blocks_for(n)), dim3(256), 0, s, m->m, which, mode, dx, dy, dz, dout, n);
 	HIP_OK(hipGetLastError());
 	HIP_OK(hipMemcpyAsync(out, dout, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
@@ -2263,6 +2321,7 @@ int nbl_driver_step_states(nbl_driver* d, nbl_step_state* states, int32_t* event
 		return rc;
 	hipStream_t s = d->stream;
 	HIP_OK(hipMemcpyAsync(dst, states, sizeof(nbl_step_state) * n, hipMemcpyHostToDevice, s));
+	(void)hipGetLastError();   // not this call's
 	hipLaunchKernelGGL(k_step_states, dim3(blocks_for(n)), dim3(256), 0, s, d->W, dst, dev_ev, dev_oc,
 		dse, dev_made, n);
 	HIP_OK(hipGetLastError());
@@ -2410,6 +2469,7 @@ int nbl_track_noalloc(nbl_device* dev, const nbl_geometry* geom, const nbl_mater
 	a.K = steps_per_launch;
 	const uint32_t g_refill = blocks_for(capacity);
 	const uint32_t g_step = std::min<uint32_t>(blocks_for(capacity), 64u * (uint32_t)dev->n_cu);
+	(void)hipGetLastError();   // not this call's
 	hipLaunchKernelGGL(k_na_init, dim3(1), dim3(64), 0, st, ctrl, capacity, (uint64_t)n);
 	Ctrl h;
 	uint32_t frame = 4;
@@ -2428,7 +2488,7 @@ int nbl_track_noalloc(nbl_device* dev, const nbl_geometry* geom, const nbl_mater
 		HIP_OK(hipMemcpyAsync(&h, ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st));
 		HIP_OK(hipStreamSynchronize(st));
 		if (steps)
-			*steps = h.steps;
+			*steps = total_steps(h);
 		if (h.det_ctr > out_cap) {
 			*n_detected = out_cap;
 			return fail(NBL_ERR_CAPACITY, "nbl_track_noalloc: more electrons detected than the output "
@@ -2439,14 +2499,14 @@ int nbl_track_noalloc(nbl_device* dev, const nbl_geometry* geom, const nbl_mater
 			break;
 		// the spill stack cannot grow here: a run whose buffer and stack are
 		// full of particles waiting for secondary slots ends with an error
-		if (h.steps == last_steps && h.det_ctr == last_det) {
+		if (total_steps(h) == last_steps && h.det_ctr == last_det) {
 			if (++stalled >= 4)
 				return fail(NBL_ERR_CAPACITY, "nbl_track_noalloc: no progress: particle buffer and spill "
 					"stack full of particles waiting for secondary-electron slots");
 		} else {
 			stalled = 0;
 		}
-		last_steps = h.steps;
+		last_steps = total_steps(h);
 		last_det = h.det_ctr;
 		frame = std::min<uint32_t>(frame * 2, 32);
 	}

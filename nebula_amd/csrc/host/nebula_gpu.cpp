@@ -170,7 +170,7 @@ void write_deposits(const std::vector<nbl_deposit>& dep, uint64_t n,
 }
 
 // Detected electrons -> 36-byte records, pixel looked up by tag.
-void write_records(const std::vector<nbl_detected>& det, uint32_t n,
+void write_records(const nbl_detected* det, uint32_t n,
 	const std::vector<nbl_record36>& pri, FILE* out, std::vector<char>& buf)
 {
 	buf.resize((size_t)n * sizeof(nbl_record36));
@@ -431,12 +431,45 @@ int run(int argc, char** argv)
 		check(nbl_driver_push(drv, part.data() + n_prescan, tags.data() + n_prescan,
 			(uint32_t)(N - n_prescan), n_prescan, &pushed), "push");
 	}
-	std::vector<nbl_detected> det(capacity);
 	std::vector<char> obuf;
 	auto last_print = std::chrono::steady_clock::now();
+	if (!deposits) {
+		// The library's frame loop: each frame's records reach the sink
+		// (written out here) while the GPU already runs the next frame.
+		struct Sink {
+			const std::vector<nbl_record36>* pri;
+			FILE* out;
+			std::vector<char>* buf;
+			size_t n_total;
+			std::chrono::steady_clock::time_point last_print;
+			bool write_failed;
+		} sk{ &pri, det_out, &obuf, N, last_print, false };
+		auto sink = [](void* user, const nbl_detected* recs, uint32_t n, uint32_t running, uint64_t queued) {
+			Sink& k = *(Sink*)user;
+			try {
+				write_records(recs, n, *k.pri, k.out, *k.buf);
+			} catch (const std::exception&) {
+				k.write_failed = true;
+				return 1;
+			}
+			const auto now = std::chrono::steady_clock::now();
+			const bool done = running == 0 && queued == 0;
+			if (done || now - k.last_print >= std::chrono::seconds(1)) {
+				std::clog << " \rProgress " << std::fixed << std::setprecision(2)
+					<< 100.0 * (1.0 - (double)queued / (double)k.n_total) << "% | Running: " << running;
+				k.last_print = now;
+			}
+			return 0;
+		};
+		const int rc = nbl_driver_run_frames(drv, frame_size, sink, &sk);
+		if (sk.write_failed)
+			throw Failure("writing detected electrons failed");
+		check(rc, "tracking");
+	}
+	std::vector<nbl_detected> det(deposits ? capacity : 0);
 	last_steps = ~0ull;
 	stalled = 0;
-	for (;;) {
+	while (deposits) {
 		uint32_t got = 0, running = 0;
 		uint64_t queued = 0, steps = 0;
 		check(nbl_driver_drain_detected(drv, det.data(), capacity, &got), "reading detected electrons");
@@ -445,7 +478,7 @@ int run(int argc, char** argv)
 		const bool done = running == 0 && queued == 0;
 		if (!done)
 			check(nbl_driver_iterate(drv, deposits ? 1 : frame_size), "iterate");
-		write_records(det, got, pri, det_out, obuf);   // overlaps the frame just launched
+		write_records(det.data(), got, pri, det_out, obuf);   // overlaps the frame just launched
 		if (deposits && !done)
 			drain_deposits();
 		const auto now = std::chrono::steady_clock::now();

@@ -167,3 +167,34 @@ def test_cascades_match_oracle(dev, capacity, K):
     print(f"detected gpu={len(det_g)} oracle={len(det_o)} common={n_common} same={n_same} only={n_only}")
     assert len(det_g) > 50
     assert n_only == 0 and n_common == n_same == len(det_o) == len(det_g)
+
+
+def test_run_frames_records_match_oracle(dev):
+    """nbl_driver_run_frames (the command line's frame loop): the records
+    handed to the sink over all frames equal the oracle's, the sink sees
+    the counts fall to zero, and a sink that fails stops the run."""
+    tris = slab_tris()
+    raw = material("si", "penn")
+    seed = 3
+    g = A.Geometry(dev, tris)
+    m = A.Material(dev, A.HostMaterial(raw))
+    pri = W.pencil_beam(3000, 1000.0)
+    d = A.Driver(dev, 4096, g, [m], seed=seed, steps_per_launch=4)
+    d.push(pri)
+    got, counts = [], []
+    d.run_frames(3, lambda recs, running, queued: (got.append(recs), counts.append((running, queued))))
+    det_g = np.concatenate(got)
+    det_o, _ = O.OracleWorld(tris, [raw]).run_philox(pri, seed, 10 ** 6, threads=8)
+    n_common, n_same, n_only = _compare_detected(det_g, det_o)
+    assert n_only == 0 and n_common == n_same == len(det_o) == len(det_g) > 100
+    assert len(counts) > 2 and counts[-1] == (0, 0)
+
+    d.push(pri, sid_base=1 << 32)
+
+    def stop(recs, running, queued):
+        raise RuntimeError("sink gave up")
+    try:
+        with pytest.raises(RuntimeError, match="sink gave up"):
+            d.run_frames(3, stop)
+    finally:
+        d.close()

@@ -6,6 +6,8 @@ on one GPU, so each reuses what the previous one left in the array: a
 larger slab run, then a smaller grating raster with device tags and a
 stream-id base, then a call whose output array is too small (an error that
 leaves the array mid-run) followed by the grating raster again."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -20,10 +22,34 @@ pytestmark = pytest.mark.gpu
 SEED = 23
 
 
-def _device_bytes(torch, arr: np.ndarray):
-    """A device copy of a numpy array's bytes (torch owns the memory)."""
-    host = np.frombuffer(np.ascontiguousarray(arr).tobytes(), np.uint8).copy()
-    return torch.from_numpy(host).to("cuda:0")
+class DeviceBytes:
+    """Device memory the test owns, from the HIP runtime the library uses
+    (the caller of the entry point allocates its inputs and outputs)."""
+
+    def __init__(self, nbytes: int, host: np.ndarray | None = None):
+        # the HIP runtime libnebula.so is bound to (symbols looked up through
+        # the library's handle resolve in its dependencies)
+        self.hip = C.CDLL(_lib.LIB_PATH)
+        assert self.hip.hipSetDevice(0) == 0
+        self.nbytes = max(1, nbytes)
+        self.ptr = C.c_void_p()
+        assert self.hip.hipMalloc(C.byref(self.ptr), C.c_size_t(self.nbytes)) == 0
+        if host is None:
+            assert self.hip.hipMemset(self.ptr, 0, C.c_size_t(self.nbytes)) == 0
+        else:
+            src = np.ascontiguousarray(host)
+            assert self.hip.hipMemcpy(self.ptr, src.ctypes.data_as(C.c_void_p), C.c_size_t(src.nbytes), 1) == 0
+
+    def read(self, nbytes: int) -> bytes:
+        out = np.empty(nbytes, np.uint8)
+        if nbytes:
+            assert self.hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(nbytes), 2) == 0
+        return out.tobytes()
+
+    def __del__(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
 
 
 def _driver_run(dev, g, mats, pri, sid_base, tags):
@@ -35,21 +61,21 @@ def _driver_run(dev, g, mats, pri, sid_base, tags):
     return det, steps
 
 
-def _noalloc_run(torch, dev, g, mats, pri, sid_base, tags, out_cap):
+def _noalloc_run(dev, g, mats, pri, sid_base, tags, out_cap):
     p = A.to_particles(pri)
-    d_pri = _device_bytes(torch, p)
-    d_tags = None if tags is None else _device_bytes(torch, np.asarray(tags, np.uint32))
-    d_out = torch.zeros(out_cap * _lib.DET_DTYPE.itemsize, dtype=torch.uint8, device="cuda:0")
-    torch.cuda.synchronize()   # the library launches on its own stream
-    n, steps = A.track_noalloc(dev, g, mats, d_pri.data_ptr(), len(p), d_out.data_ptr(), out_cap,
-                               tags_dev=None if d_tags is None else d_tags.data_ptr(), sid_base=sid_base,
+    d_pri = DeviceBytes(p.nbytes, p)
+    d_tags = None if tags is None else DeviceBytes(4 * len(tags), np.asarray(tags, np.uint32))
+    rec = _lib.DET_DTYPE.itemsize
+    d_out = DeviceBytes(out_cap * rec)
+    # the copies above are synchronous: the inputs are in place before the call
+    n, steps = A.track_noalloc(dev, g, mats, d_pri.ptr.value, len(p), d_out.ptr.value, out_cap,
+                               tags_dev=None if d_tags is None else d_tags.ptr.value, sid_base=sid_base,
                                seed=SEED, capacity=4096, steps_per_launch=4)
-    out = np.frombuffer(d_out[:n * _lib.DET_DTYPE.itemsize].cpu().numpy().tobytes(), _lib.DET_DTYPE)
+    out = np.frombuffer(d_out.read(n * rec), _lib.DET_DTYPE)
     return out, steps
 
 
 def test_noalloc_equals_driver_run():
-    torch = pytest.importorskip("torch")
     dev = A.Device(0)
     si, pmma = material("si"), material("pmma")
     m_si, m_pmma = A.Material(dev, A.HostMaterial(si)), A.Material(dev, A.HostMaterial(pmma))
@@ -61,7 +87,7 @@ def test_noalloc_equals_driver_run():
     for name, (g, mats), pri, sid_base, tags in cases:
         print(f"noalloc case: {name}", flush=True)
         exp, steps_e = _driver_run(dev, g, mats, pri, sid_base, tags)
-        got, steps_g = _noalloc_run(torch, dev, g, mats, pri, sid_base, tags, out_cap=4 * len(pri) + 1000)
+        got, steps_g = _noalloc_run(dev, g, mats, pri, sid_base, tags, out_cap=4 * len(pri) + 1000)
         n_common, n_same, n_only = _compare_detected(got, exp)
         print(f"{name}: {len(got)} records, {steps_g} steps", flush=True)
         assert len(exp) > 50
@@ -71,9 +97,9 @@ def test_noalloc_equals_driver_run():
     # reused from a fresh start
     name, (g, mats), pri, sid_base, tags = cases[1]
     with pytest.raises(A.NebulaError, match="more electrons detected"):
-        _noalloc_run(torch, dev, g, mats, pri, sid_base, tags, out_cap=10)
+        _noalloc_run(dev, g, mats, pri, sid_base, tags, out_cap=10)
     exp, steps_e = _driver_run(dev, g, mats, pri, sid_base, tags)
-    got, steps_g = _noalloc_run(torch, dev, g, mats, pri, sid_base, tags, out_cap=4 * len(pri) + 1000)
+    got, steps_g = _noalloc_run(dev, g, mats, pri, sid_base, tags, out_cap=4 * len(pri) + 1000)
     n_common, n_same, n_only = _compare_detected(got, exp)
     assert n_only == 0 and n_common == n_same == len(exp) == len(got) and steps_g == steps_e
 
