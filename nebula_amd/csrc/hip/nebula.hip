@@ -693,6 +693,19 @@ __global__ __launch_bounds__(256) void k_inel(StepArgs a)
 	}
 }
 
+// Queue entries of one push: tag (tag_base + i, unless given), stream id
+// sid_base + i and world number.
+__global__ __launch_bounds__(256) void k_queue_ids(uint32_t* tag, uint64_t* sid, uint16_t* world, uint32_t n,
+	int fill_tags, uint32_t tag_base, uint64_t sid_base, uint16_t w)
+{
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		if (fill_tags)
+			tag[i] = tag_base + i;
+		sid[i] = sid_base + i;
+		world[i] = w;
+	}
+}
+
 __global__ void k_reset_det(Ctrl* c)
 {
 	if (threadIdx.x == 0 && blockIdx.x == 0)
@@ -1502,18 +1515,18 @@ int nbl_driver_push_world(nbl_driver* d, const nbl_particle28* particles, const 
 		d->h_ctrl->queue_n = unread;
 		d->h_ctrl->queue_next = 0;
 	}
-	std::vector<uint32_t> t(n);
-	std::vector<uint64_t> s(n);
-	for (uint32_t i = 0; i < n; ++i) {
-		t[i] = tags ? tags[i] : d->next_tag + i;
-		s[i] = sid_base + i;
+	// the particles (and given tags) are copied; the running tags, stream
+	// ids and world numbers are written on the device
+	HIP_OK(hipMemcpy(d->q_part + d->q_n, particles, sizeof(nbl_particle28) * n, hipMemcpyHostToDevice));
+	if (tags)
+		HIP_OK(hipMemcpy(d->q_tag + d->q_n, tags, sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+	if (n) {
+		hipLaunchKernelGGL(k_queue_ids, dim3(blocks_for(n)), dim3(256), 0, d->stream, d->q_tag + d->q_n,
+			d->q_sid + d->q_n, d->q_world + d->q_n, n, tags ? 0 : 1, d->next_tag, sid_base, (uint16_t)world);
+		HIP_OK(hipGetLastError());
+		HIP_OK(hipStreamSynchronize(d->stream));
 	}
 	d->next_tag += n;
-	HIP_OK(hipMemcpy(d->q_part + d->q_n, particles, sizeof(nbl_particle28) * n, hipMemcpyHostToDevice));
-	HIP_OK(hipMemcpy(d->q_tag + d->q_n, t.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-	HIP_OK(hipMemcpy(d->q_sid + d->q_n, s.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice));
-	const std::vector<uint16_t> wv(n, (uint16_t)world);
-	HIP_OK(hipMemcpy(d->q_world + d->q_n, wv.data(), sizeof(uint16_t) * n, hipMemcpyHostToDevice));
 	d->q_n += n;
 	const uint64_t qn = d->q_n;
 	HIP_OK(hipMemcpy(&d->ctrl->queue_n, &qn, sizeof(uint64_t), hipMemcpyHostToDevice));

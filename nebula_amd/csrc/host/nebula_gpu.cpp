@@ -331,15 +331,14 @@ int run(int argc, char** argv)
 		<< "Loaded " << pri.size() << " primaries.\n"
 		<< "Loaded " << hmats.size() << " materials.\n\n" << std::flush;
 
-	// Primary i (after sort and shuffle) has tag i and stream id i.
+	// Primary i (after sort and shuffle) has tag i and stream id i (the
+	// driver's running tags: the two pushes below are its first).
 	const size_t N = pri.size();
 	std::vector<nbl_particle28> part(N);
-	std::vector<uint32_t> tags(N);
 	for (size_t i = 0; i < N; ++i) {
 		std::memcpy(part[i].pos, pri[i].pos, sizeof part[i].pos);
 		std::memcpy(part[i].dir, pri[i].dir, sizeof part[i].dir);
 		part[i].kin_energy = pri[i].kin_energy;
-		tags[i] = (uint32_t)i;
 	}
 
 	// ------------------------------------------------------------ device
@@ -387,7 +386,7 @@ int run(int argc, char** argv)
 	std::vector<std::pair<uint32_t, uint32_t>> stats;
 	{
 		uint32_t pushed = 0;
-		check(nbl_driver_push(drv, part.data(), tags.data(), n_prescan, 0, &pushed), "push");
+		check(nbl_driver_push(drv, part.data(), nullptr, n_prescan, 0, &pushed), "push");
 		stats.push_back({ pushed, 0 });
 	}
 	uint64_t last_steps = ~0ull;
@@ -428,8 +427,8 @@ int run(int argc, char** argv)
 	timer.start();
 	if (N > n_prescan) {
 		uint32_t pushed = 0;
-		check(nbl_driver_push(drv, part.data() + n_prescan, tags.data() + n_prescan,
-			(uint32_t)(N - n_prescan), n_prescan, &pushed), "push");
+		check(nbl_driver_push(drv, part.data() + n_prescan, nullptr, (uint32_t)(N - n_prescan), n_prescan,
+			&pushed), "push");
 	}
 	std::vector<char> obuf;
 	auto last_print = std::chrono::steady_clock::now();
