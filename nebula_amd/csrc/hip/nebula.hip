@@ -426,6 +426,10 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	// multi-world variant has the registers and ran 3 % slower with it.
 	constexpr bool SID_LDS = !SC && !MULTI;
 	__shared__ uint64_t s_sid[SID_LDS ? 256 : 1];
+	// With the stream id wait the particle's tag, step and flags (the walk
+	// needs none of them): three VGPRs fewer across the step loop.
+	__shared__ uint32_t s_aux[SID_LDS ? 3 * 256 : 1];
+	uint32_t* const aux = s_aux + (SID_LDS ? threadIdx.x : 0u);
 
 	int* path = s_path + threadIdx.x;
 	int* owner_at = s_owner + 4 * (threadIdx.x & ~63u);
@@ -462,8 +466,12 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 				Wl.nmats = wd.nmats;
 			}
 		}
-		if (SID_LDS)
+		if (SID_LDS) {
 			s_sid[threadIdx.x] = s.sid;
+			aux[0] = s.tag;
+			aux[256] = s.step;
+			aux[512] = s.flags;
+		}
 		bool run = valid, det = false, to_inel = false, dead = false;
 		float rem = SKIP ? s.safe : -1.0f;
 		for (uint32_t k = 0; k < K; ++k) {
@@ -477,14 +485,18 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 			float distance = 0.0f;
 			PathInfo pinfo;
 			if (run) {
-				if (s.flags & FL_PEND_DET) {
+				const uint32_t fl = SID_LDS ? aux[512] : s.flags;
+				if (fl & FL_PEND_DET) {
 					oc = DETECTED;
-				} else if (s.flags & FL_PEND_INEL) {
+				} else if (fl & FL_PEND_INEL) {
 					// an inelastic event that found no secondary slot earlier
-					s.flags &= ~FL_PEND_INEL;
+					if (SID_LDS)
+						aux[512] = fl & ~FL_PEND_INEL;
+					else
+						s.flags = fl & ~FL_PEND_INEL;
 					to_inel = true;
 				} else {
-					rng_start(rng, W.key0, W.key1, SID_LDS ? s_sid[threadIdx.x] : s.sid, s.step);
+					rng_start(rng, W.key0, W.key1, SID_LDS ? s_sid[threadIdx.x] : s.sid, SID_LDS ? aux[256] : s.step);
 					mat0 = s.mat;
 					oc = step_plan(W, s, rng, nd, distance, type, pinfo);
 					++my_steps;
@@ -534,13 +546,19 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 					run = false;
 				} else if (to_inel) {
 					run = false;
+				} else if (SID_LDS) {
+					aux[256] += 1u;
 				} else {
 					s.step++;
 				}
 			}
 		}
-		if (SID_LDS)
+		if (SID_LDS) {
 			s.sid = s_sid[threadIdx.x];
+			s.tag = aux[0];
+			s.step = aux[256];
+			s.flags = aux[512];
+		}
 		// the multi-world kernel moves particles without a safety: clear it
 		s.safe = rem;
 		bool keep = valid && !dead;
