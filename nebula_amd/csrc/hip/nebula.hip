@@ -600,10 +600,11 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	count_steps(C, my_steps);
 }
 
-// The plain variant runs at 5 waves per SIMD (96 VGPRs, a 40-byte spill):
-// measured faster than 6 waves (80 VGPRs, 84-byte spill) or 7 (72 VGPRs,
-// 128-byte spill).  The scored and multi-world variants would spill, so
-// they are left free.
+// The plain variant runs at 5 waves per SIMD (96 VGPRs, 6 of them spilled,
+// 28 bytes of scratch per lane, hipcc -Rpass-analysis=kernel-resource-usage):
+// measured faster than 6 waves (80 VGPRs, 2 % slower with the tag, step and
+// flags in LDS, 11 % before) or 7.  The scored and multi-world variants
+// would spill, so they are left free.
 #ifndef NBL_PLAIN_WAVES
 #define NBL_PLAIN_WAVES 5
 #endif
