@@ -39,14 +39,18 @@
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <iomanip>
 #include <iostream>
 #include <map>
+#include <mutex>
 #include <random>
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -169,9 +173,9 @@ void write_deposits(const std::vector<nbl_deposit>& dep, uint64_t n,
 		throw Failure("writing energy deposits failed");
 }
 
-// Detected electrons -> 36-byte records, pixel looked up by tag.
-void write_records(const nbl_detected* det, uint32_t n,
-	const std::vector<nbl_record36>& pri, FILE* out, std::vector<char>& buf)
+// Detected electrons -> 36-byte records in `buf`, pixel looked up by tag.
+void pack_records(const nbl_detected* det, uint32_t n, const std::vector<nbl_record36>& pri,
+	std::vector<char>& buf)
 {
 	buf.resize((size_t)n * sizeof(nbl_record36));
 	for (uint32_t i = 0; i < n; ++i) {
@@ -183,9 +187,95 @@ void write_records(const nbl_detected* det, uint32_t n,
 		r.pixel[1] = pri[det[i].tag].pixel[1];
 		std::memcpy(buf.data() + (size_t)i * sizeof r, &r, sizeof r);
 	}
+}
+
+void write_records(const nbl_detected* det, uint32_t n,
+	const std::vector<nbl_record36>& pri, FILE* out, std::vector<char>& buf)
+{
+	pack_records(det, n, pri, buf);
 	if (n && std::fwrite(buf.data(), sizeof(nbl_record36), n, out) != n)
 		throw Failure("writing detected electrons failed");
 }
+
+// Writes blocks of output records on a thread of its own, so that a slow
+// reader of the output (a pipe) holds up the frame loop only once
+// `max_pending` blocks wait to be written.  Written blocks are kept for
+// reuse (at most `max_pending` of them).
+class RecordWriter
+{
+public:
+	RecordWriter(FILE* out, size_t max_pending) : out_(out), max_pending_(max_pending) {}
+	~RecordWriter() { finish(); }
+
+	// An empty block to fill: a written one if there is one.
+	std::vector<char> take()
+	{
+		std::lock_guard<std::mutex> g(m_);
+		if (spare_.empty())
+			return {};
+		std::vector<char> b = std::move(spare_.back());
+		spare_.pop_back();
+		return b;
+	}
+
+	// Queue a filled block; false once a write has failed.
+	bool put(std::vector<char>&& b)
+	{
+		std::unique_lock<std::mutex> g(m_);
+		room_.wait(g, [&] { return pending_.size() < max_pending_ || failed_; });
+		if (failed_)
+			return false;
+		pending_.push_back(std::move(b));
+		work_.notify_one();
+		return true;
+	}
+
+	// Write everything queued and stop the thread; false if a write failed.
+	bool finish()
+	{
+		{
+			std::lock_guard<std::mutex> g(m_);
+			done_ = true;
+		}
+		work_.notify_one();
+		if (th_.joinable())
+			th_.join();
+		return !failed_;
+	}
+
+private:
+	void loop()
+	{
+		std::unique_lock<std::mutex> g(m_);
+		for (;;) {
+			work_.wait(g, [&] { return !pending_.empty() || done_; });
+			if (pending_.empty())
+				return;
+			std::vector<char> b = std::move(pending_.front());
+			pending_.pop_front();
+			if (!failed_) {
+				g.unlock();
+				const bool ok = std::fwrite(b.data(), 1, b.size(), out_) == b.size();
+				g.lock();
+				failed_ = failed_ || !ok;
+			}
+			if (spare_.size() < max_pending_) {
+				b.clear();
+				spare_.push_back(std::move(b));
+			}
+			room_.notify_all();
+		}
+	}
+
+	FILE* out_;
+	const size_t max_pending_;
+	std::mutex m_;
+	std::condition_variable work_, room_;
+	std::deque<std::vector<char>> pending_;
+	std::vector<std::vector<char>> spare_;
+	bool done_ = false, failed_ = false;
+	std::thread th_{ [this] { loop(); } };   // last: starts once the members above exist
+};
 
 int run(int argc, char** argv)
 {
@@ -434,19 +524,27 @@ int run(int argc, char** argv)
 	auto last_print = std::chrono::steady_clock::now();
 	if (!deposits) {
 		// The library's frame loop: each frame's records reach the sink
-		// (written out here) while the GPU already runs the next frame.
+		// while the GPU already runs the next frame; the sink converts them
+		// and hands them to the writer thread.
+		RecordWriter writer(det_out, 4);
 		struct Sink {
 			const std::vector<nbl_record36>* pri;
-			FILE* out;
-			std::vector<char>* buf;
+			RecordWriter* writer;
 			size_t n_total;
 			std::chrono::steady_clock::time_point last_print;
 			bool write_failed;
-		} sk{ &pri, det_out, &obuf, N, last_print, false };
+		} sk{ &pri, &writer, N, last_print, false };
 		auto sink = [](void* user, const nbl_detected* recs, uint32_t n, uint32_t running, uint64_t queued) {
 			Sink& k = *(Sink*)user;
 			try {
-				write_records(recs, n, *k.pri, k.out, *k.buf);
+				if (n) {
+					std::vector<char> b = k.writer->take();
+					pack_records(recs, n, *k.pri, b);
+					if (!k.writer->put(std::move(b))) {
+						k.write_failed = true;
+						return 1;
+					}
+				}
 			} catch (const std::exception&) {
 				k.write_failed = true;
 				return 1;
@@ -461,6 +559,8 @@ int run(int argc, char** argv)
 			return 0;
 		};
 		const int rc = nbl_driver_run_frames(drv, frame_size, sink, &sk);
+		if (!writer.finish())
+			sk.write_failed = true;
 		if (sk.write_failed)
 			throw Failure("writing detected electrons failed");
 		check(rc, "tracking");
