@@ -363,6 +363,16 @@ int nbl_driver_get_info(nbl_driver* d, nbl_driver_info* out);
 int nbl_driver_set_timing(nbl_driver* d, int enable);
 int nbl_driver_timing(nbl_driver* d, double* step_ms_total, uint64_t* step_launches);
 
+/* Page-locked host memory on `dev`'s runtime (hipHostMalloc).  A push from
+ * it, or a drain into it, is one DMA at the link's rate instead of the
+ * runtime's copy through bounce buffers.  The reference stages its pushes
+ * and drains through cudaMallocHost buffers of its own
+ * (drivers/gpu/gpu_driver.inl:81-83, 208-210); here the caller writes its
+ * particles straight into such memory.  *ptr is null for bytes == 0.
+ * Release with nbl_host_free. */
+int nbl_host_alloc(nbl_device* dev, uint64_t bytes, void** ptr);
+int nbl_host_free(void* ptr);
+
 /* Measurement aid (no reference counterpart): free and total device memory
  * of `dev` (hipMemGetInfo), for reporting a run's footprint. */
 int nbl_device_memory(nbl_device* dev, uint64_t* free_bytes, uint64_t* total_bytes);

@@ -173,6 +173,8 @@ _SIGS = {
                                   C.c_void_p, C.c_void_p, C.c_uint32]),
     "nbl_device_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, _p(C.c_float)]),
     "nbl_device_memory": (C.c_int, [C.c_void_p, _p(C.c_uint64), _p(C.c_uint64)]),
+    "nbl_host_alloc": (C.c_int, [C.c_void_p, C.c_uint64, _p(C.c_void_p)]),
+    "nbl_host_free": (C.c_int, [C.c_void_p]),
     "nbl_driver_step_states": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_uint32]),
     "nbl_driver_set_material_params": (C.c_int, [C.c_void_p, C.c_int32, C.c_float, C.c_float, C.c_float]),

@@ -144,6 +144,21 @@ class Device:
     def synchronize(self):
         check(lib().nbl_device_synchronize(self._h))
 
+    def host_array(self, n: int, dtype=None) -> np.ndarray:
+        """An uninitialised array of n elements (default: the 28-byte
+        particle records) in page-locked host memory (nbl_host_alloc):
+        Driver.push from it is one DMA.  The memory is released when the
+        array and every view of it are gone."""
+        dtype = np.dtype(PART_DTYPE if dtype is None else dtype)
+        nbytes = int(n) * dtype.itemsize
+        if nbytes == 0:
+            return np.zeros(0, dtype)
+        p = C.c_void_p()
+        check(lib().nbl_host_alloc(self._h, nbytes, C.byref(p)))
+        buf = (C.c_char * nbytes).from_address(p.value)
+        weakref.finalize(buf, lib().nbl_host_free, C.c_void_p(p.value))
+        return np.frombuffer(buf, dtype)
+
     def close(self):
         if self._h:
             for group in (self._drivers, self._others):

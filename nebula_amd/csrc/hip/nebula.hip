@@ -1392,8 +1392,12 @@ int nbl_driver_create(nbl_device* dev, uint32_t capacity, const nbl_geometry* ge
 	}
 	if ((rc = alloc_bufs(d->buf[0], d->buf_mem[0], d->slots))) return cleanup(rc);
 	if ((rc = alloc_bufs(d->buf[1], d->buf_mem[1], d->slots))) return cleanup(rc);
-	// the spill stack starts at four buffers' worth and doubles when needed
-	d->spill_cap = std::max<uint32_t>(1024u, std::min<uint64_t>(4ull * d->slots, 1ull << 28));
+	// The spill stack starts at eight buffers' worth (60 bytes a slot: 2 GB
+	// at 2^22 slots) and doubles when needed.  A scan of 300 eV primaries
+	// filling 2^22 slots spills more than twice the buffer in its first
+	// iterations; each doubling costs a copy and a reallocation with the
+	// stream idle.
+	d->spill_cap = std::max<uint32_t>(1024u, std::min<uint64_t>(8ull * d->slots, 1ull << 28));
 	if ((rc = alloc_bufs(d->spill, d->spill_mem, d->spill_cap))) return cleanup(rc);
 	if ((rc = alloc_bufs(d->inel, d->inel_mem, d->slots))) return cleanup(rc);
 	if ((rc = dev_alloc(&d->ctrl, 1))) return cleanup(rc);
@@ -2364,6 +2368,33 @@ int nbl_driver_step_states(nbl_driver* d, nbl_step_state* states, int32_t* event
 	HIP_OK(hipMemcpyAsync(se_made, dev_made, 4 * (size_t)n, hipMemcpyDeviceToHost, s));
 	HIP_OK(hipStreamSynchronize(s));
 	hipFree(dst); hipFree(dse); hipFree(dev_ev); hipFree(dev_oc); hipFree(dev_made);
+	return NBL_OK;
+}
+
+int nbl_host_alloc(nbl_device* dev, uint64_t bytes, void** ptr)
+{
+	clear_error();
+	if (!dev || !ptr)
+		return fail(NBL_ERR_ARG, "invalid host allocation arguments");
+	*ptr = nullptr;
+	if (!bytes)
+		return NBL_OK;
+	int rc = set_device(dev);
+	if (rc) return rc;
+	void* p = nullptr;
+	if (hipHostMalloc(&p, (size_t)bytes, hipHostMallocDefault) != hipSuccess || !p) {
+		(void)hipGetLastError();
+		return fail(NBL_ERR_ALLOC, "page-locked host memory of " + std::to_string(bytes) + " bytes");
+	}
+	*ptr = p;
+	return NBL_OK;
+}
+
+int nbl_host_free(void* ptr)
+{
+	clear_error();
+	if (ptr)
+		HIP_OK(hipHostFree(ptr));
 	return NBL_OK;
 }
 

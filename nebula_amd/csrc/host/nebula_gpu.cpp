@@ -174,19 +174,37 @@ void write_deposits(const std::vector<nbl_deposit>& dep, uint64_t n,
 }
 
 // Detected electrons -> 36-byte records in `buf`, pixel looked up by tag.
+// The lookups are random reads into the primaries, so a large frame is
+// split over several threads.
 void pack_records(const nbl_detected* det, uint32_t n, const std::vector<nbl_record36>& pri,
 	std::vector<char>& buf)
 {
 	buf.resize((size_t)n * sizeof(nbl_record36));
-	for (uint32_t i = 0; i < n; ++i) {
-		nbl_record36 r;
-		std::memcpy(r.pos, det[i].pos, sizeof r.pos);
-		std::memcpy(r.dir, det[i].dir, sizeof r.dir);
-		r.kin_energy = det[i].kin_energy;
-		r.pixel[0] = pri[det[i].tag].pixel[0];
-		r.pixel[1] = pri[det[i].tag].pixel[1];
-		std::memcpy(buf.data() + (size_t)i * sizeof r, &r, sizeof r);
+	auto pack = [&](uint32_t lo, uint32_t hi) {
+		for (uint32_t i = lo; i < hi; ++i) {
+			nbl_record36 r;
+			std::memcpy(r.pos, det[i].pos, sizeof r.pos);
+			std::memcpy(r.dir, det[i].dir, sizeof r.dir);
+			r.kin_energy = det[i].kin_energy;
+			r.pixel[0] = pri[det[i].tag].pixel[0];
+			r.pixel[1] = pri[det[i].tag].pixel[1];
+			std::memcpy(buf.data() + (size_t)i * sizeof r, &r, sizeof r);
+		}
+	};
+	const uint32_t per_thread = 1u << 16;
+	const uint32_t nt = std::min<uint32_t>({ (n + per_thread - 1) / per_thread, 16u,
+		std::max(1u, std::thread::hardware_concurrency()) });
+	if (nt <= 1) {
+		pack(0, n);
+		return;
 	}
+	std::vector<std::thread> th;
+	const uint32_t chunk = (n + nt - 1) / nt;
+	for (uint32_t t = 1; t < nt; ++t)
+		th.emplace_back(pack, std::min(n, t * chunk), std::min(n, (t + 1) * chunk));
+	pack(0, std::min(n, chunk));
+	for (auto& x : th)
+		x.join();
 }
 
 void write_records(const nbl_detected* det, uint32_t n,
@@ -421,20 +439,24 @@ int run(int argc, char** argv)
 		<< "Loaded " << pri.size() << " primaries.\n"
 		<< "Loaded " << hmats.size() << " materials.\n\n" << std::flush;
 
+	// ------------------------------------------------------------ device
+	nbl_device* dev = nullptr;
+	check(nbl_device_open(device, &dev), "opening the GPU");
+	std::clog << "Using GPU " << device << "\n" << std::endl;
+
 	// Primary i (after sort and shuffle) has tag i and stream id i (the
-	// driver's running tags: the two pushes below are its first).
+	// driver's running tags: the two pushes below are its first).  The
+	// particles are written into page-locked memory, so that each push is
+	// one DMA.
 	const size_t N = pri.size();
-	std::vector<nbl_particle28> part(N);
+	void* part_mem = nullptr;
+	check(nbl_host_alloc(dev, sizeof(nbl_particle28) * N, &part_mem), "page-locked memory for the primaries");
+	nbl_particle28* part = static_cast<nbl_particle28*>(part_mem);
 	for (size_t i = 0; i < N; ++i) {
 		std::memcpy(part[i].pos, pri[i].pos, sizeof part[i].pos);
 		std::memcpy(part[i].dir, pri[i].dir, sizeof part[i].dir);
 		part[i].kin_energy = pri[i].kin_energy;
 	}
-
-	// ------------------------------------------------------------ device
-	nbl_device* dev = nullptr;
-	check(nbl_device_open(device, &dev), "opening the GPU");
-	std::clog << "Using GPU " << device << "\n" << std::endl;
 	nbl_geometry* geom = nullptr;
 	check(nbl_geometry_upload(dev, octree, n_octree, sorted, n_sorted, center, halfsize, &geom),
 		"uploading the geometry");
@@ -476,7 +498,7 @@ int run(int argc, char** argv)
 	std::vector<std::pair<uint32_t, uint32_t>> stats;
 	{
 		uint32_t pushed = 0;
-		check(nbl_driver_push(drv, part.data(), nullptr, n_prescan, 0, &pushed), "push");
+		check(nbl_driver_push(drv, part, nullptr, n_prescan, 0, &pushed), "push");
 		stats.push_back({ pushed, 0 });
 	}
 	uint64_t last_steps = ~0ull;
@@ -517,7 +539,7 @@ int run(int argc, char** argv)
 	timer.start();
 	if (N > n_prescan) {
 		uint32_t pushed = 0;
-		check(nbl_driver_push(drv, part.data() + n_prescan, nullptr, (uint32_t)(N - n_prescan), n_prescan,
+		check(nbl_driver_push(drv, part + n_prescan, nullptr, (uint32_t)(N - n_prescan), n_prescan,
 			&pushed), "push");
 	}
 	std::vector<char> obuf;
@@ -608,6 +630,7 @@ int run(int argc, char** argv)
 	timer.print(std::clog);
 
 	nbl_driver_destroy(drv);
+	nbl_host_free(part_mem);
 	for (nbl_material* m : dmats)
 		nbl_material_destroy(m);
 	nbl_geometry_destroy(geom);

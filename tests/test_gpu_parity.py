@@ -198,3 +198,24 @@ def test_run_frames_records_match_oracle(dev):
             d.run_frames(3, stop)
     finally:
         d.close()
+
+
+def test_push_from_page_locked_memory(dev):
+    """Primaries written into nbl_host_alloc memory (Device.host_array) and
+    pushed from there track to the oracle's records, as from pageable
+    memory."""
+    tris = slab_tris()
+    raw = material("si", "penn")
+    seed = 4
+    g = A.Geometry(dev, tris)
+    d = A.Driver(dev, 4096, g, [A.Material(dev, A.HostMaterial(raw))], seed=seed, steps_per_launch=4)
+    pri = W.pencil_beam(300, 1000.0)
+    pinned = dev.host_array(len(pri))
+    pinned[:] = A.to_particles(pri)
+    assert d.push(pinned) == len(pri)
+    det_g = d.run(100000)
+    d.close()
+    det_o, _ = O.OracleWorld(tris, [raw]).run_philox(pri, seed, 100000, threads=8)
+    n_common, n_same, n_only = _compare_detected(det_g, det_o)
+    assert n_only == 0 and n_common == n_same == len(det_o) == len(det_g) > 50
+    assert len(dev.host_array(0)) == 0
