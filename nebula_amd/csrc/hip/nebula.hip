@@ -1404,6 +1404,19 @@ int nbl_driver_create(nbl_device* dev, uint32_t capacity, const nbl_geometry* ge
 	if (hipHostMalloc((void**)&d->h_ctrl, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess)
 		return cleanup(fail(NBL_ERR_ALLOC, "hipHostMalloc failed"));
 	if ((rc = dev_alloc(&d->det, d->det_cap))) return cleanup(rc);
+	// Pinned staging for the frame loop's record copies, allocated here as
+	// the reference's driver allocates its pinned buffers
+	// (drivers/gpu/gpu_driver.inl:81-83), with one device-to-host copy into
+	// it on the driver's stream: the first large device-to-host copy on a
+	// stream waits several ms inside hipMemcpyAsync before it starts, which
+	// would otherwise idle the GPU at the end of the first frame.
+	if (hipHostMalloc((void**)&d->h_stage, sizeof(nbl_detected) * d->det_cap, hipHostMallocDefault) != hipSuccess) {
+		d->h_stage = nullptr;
+		return cleanup(fail(NBL_ERR_ALLOC, "pinned staging for detected records"));
+	}
+	if (hipMemcpyAsync(d->h_stage, d->det, sizeof(nbl_detected) * std::min<size_t>(d->det_cap, 1u << 16),
+			hipMemcpyDeviceToHost, d->stream) != hipSuccess || hipStreamSynchronize(d->stream) != hipSuccess)
+		return cleanup(fail(NBL_ERR_DEVICE, "device-to-host copy into the staging buffer failed"));
 	if ((rc = dev_alloc(&d->d_mats, (size_t)std::max(1, n_mats)))) return cleanup(rc);
 	std::vector<Material> hm((size_t)std::max(1, n_mats));
 	for (int i = 0; i < n_mats; ++i)
