@@ -116,6 +116,10 @@ struct StepArgs {
 	uint32_t slots;
 	uint32_t spill_cap;
 	uint32_t K;
+	// world-per-wave variant: the coming launch's particles grouped by
+	// world (k_world_place); k_step's wave takes perm[i] for index i
+	const uint32_t* perm;
+	const uint32_t* perm_n;   // its padded length
 };
 
 __device__ __forceinline__ State load_state(const Bufs& b, uint32_t i)
@@ -280,6 +284,178 @@ __global__ void k_ctrl(Ctrl* c, uint32_t slots, uint32_t spill_cap)
 	c->stepped = 1;
 }
 
+// Index i (0 .. n_in) of the coming step launch: survivors first, then the
+// secondaries down from the top of the buffer.
+__device__ __forceinline__ uint32_t in_slot(uint32_t i, uint32_t n_surv, uint32_t slots)
+{
+	return (i < n_surv) ? i : (slots - 1u - (i - n_surv));
+}
+
+// Grouping by world for the world-per-wave step kernel, in three kernels:
+// k_world_group<false> counts the coming launch's particles per world;
+// k_world_scan turns the counts into the worlds' starts, each a multiple
+// of 64 so that every step wave holds one world, marks the padding after
+// each world's range empty (~0u) and clears the counters;
+// k_world_group<true> writes each index into its world's range, a wave's
+// lanes of one world together, so that the step kernel's waves read runs
+// of neighbouring slots.  wg holds counts [0, nw), starts [nw, 2 nw), fill
+// levels [2 nw, 3 nw) and the padded length of the index array at 3 nw.
+// Exclusive prefix of the counts into the starts (each padded to a
+// multiple of 64, the padding marked ~0u), the padded length at wg[3 nw],
+// counters and fill levels cleared for the next launch.
+__device__ void world_scan(uint32_t nw, uint32_t* __restrict__ wg, uint32_t* __restrict__ perm)
+{
+	__shared__ uint32_t s_v[256];
+	__shared__ uint32_t s_carry;
+	if (threadIdx.x == 0)
+		s_carry = 0;
+	__syncthreads();
+	for (uint32_t base = 0; base < nw; base += blockDim.x) {
+		const uint32_t i = base + threadIdx.x;
+		const uint32_t n = (i < nw) ? wg[i] : 0u;
+		const uint32_t v = (n + 63u) & ~63u;
+		s_v[threadIdx.x] = v;
+		__syncthreads();
+		for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
+			const uint32_t add = (threadIdx.x >= o) ? s_v[threadIdx.x - o] : 0u;
+			__syncthreads();
+			s_v[threadIdx.x] += add;
+			__syncthreads();
+		}
+		if (i < nw) {
+			const uint32_t start = s_carry + s_v[threadIdx.x] - v;
+			wg[nw + i] = start;
+			wg[i] = 0;
+			wg[2 * nw + i] = 0;
+			for (uint32_t k = n; k < v; ++k)
+				perm[start + k] = ~0u;
+		}
+		__syncthreads();
+		if (threadIdx.x == blockDim.x - 1)
+			s_carry += s_v[threadIdx.x];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		wg[3 * nw] = s_carry;
+}
+
+// Each block takes a contiguous chunk of the indices (whole waves) and
+// counts them per world in LDS first (one LDS atomic per world present in
+// a wave), so that the counters in memory see one atomic per world present
+// in a block: with one per wave, the waves' atomics on the few counters of
+// the worlds in flight serialised in L2 (165 us per pass on the 64-scan
+// batch).  PLACE: the block reserves its range of each world with those
+// atomics and a second sweep writes the indices from LDS offsets.  Worlds
+// beyond NW_LDS are counted with one atomic per wave instead.
+constexpr uint32_t NW_LDS = 1024;
+template <bool PLACE>
+__global__ __launch_bounds__(256) void k_world_group(const Ctrl* c, Bufs b0, Bufs b1, uint32_t slots,
+	uint32_t nw, uint32_t* __restrict__ wg, uint32_t* __restrict__ perm)
+{
+	__shared__ uint32_t s_h[NW_LDS];
+	const uint32_t n_surv = c->n_surv, n_in = n_surv + c->n_se;
+	const Bufs& b = c->buf_in ? b1 : b0;
+	const uint32_t lane = __lane_id();
+	const bool lds = nw <= NW_LDS;
+	// this block's chunk: whole waves
+	const uint32_t per = ((n_in + gridDim.x - 1) / gridDim.x + 63u) & ~63u;
+	const uint32_t lo = blockIdx.x * per, hi = min(n_in, lo + per);
+	if (lds) {
+		for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x)
+			s_h[w] = 0;
+		__syncthreads();
+	}
+	// sweep 0 counts; with PLACE, sweep 1 writes
+	for (int sweep = 0; sweep < (PLACE ? 2 : 1); ++sweep) {
+		if (sweep == 1) {
+			__syncthreads();
+			for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x)
+				if (s_h[w])
+					s_h[w] = wg[nw + w] + atomicAdd(&wg[2 * nw + w], s_h[w]);
+			__syncthreads();
+		}
+		for (uint32_t w0 = lo + (threadIdx.x & ~63u); w0 < hi; w0 += blockDim.x) {
+			const uint32_t i = w0 + lane;
+			const bool valid = i < hi;
+			uint32_t world = 0;
+			if (valid)
+				world = min(fld_u(b, F_FLAGS)[in_slot(i, n_surv, slots)] >> FL_WORLD_SHIFT, nw - 1u);
+			uint64_t left = __ballot(valid);
+			while (left) {
+				const int first = __builtin_ctzll(left);
+				const uint32_t w = __builtin_amdgcn_readlane(world, first);
+				const uint64_t m = __ballot(valid && world == w);
+				left &= ~m;
+				const uint32_t cnt = (uint32_t)__popcll(m);
+				if (sweep == 0) {
+					if ((int)lane == first) {
+						if (lds)
+							atomicAdd(&s_h[w], cnt);
+						else
+							atomicAdd(&wg[w], cnt);
+					}
+				} else {
+					uint32_t base = 0;
+					if ((int)lane == first)
+						base = atomicAdd(&s_h[w], cnt);
+					base = __builtin_amdgcn_readlane(base, first);
+					if ((m >> lane) & 1ull)
+						perm[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+				}
+			}
+		}
+	}
+	if (!PLACE) {
+		if (lds) {
+			__syncthreads();
+			for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x)
+				if (s_h[w])
+					atomicAdd(&wg[w], s_h[w]);
+		}
+	}
+}
+
+// One block: the counts into starts (a separate launch: a count pass
+// whose last block did this needed a device-scope fence in every block,
+// which on this multi-XCD GPU writes back the L2, 67 us against 9 + 5).
+__global__ __launch_bounds__(256) void k_world_scan(uint32_t nw, uint32_t* __restrict__ wg,
+	uint32_t* __restrict__ perm)
+{
+	world_scan(nw, wg, perm);
+}
+
+// The placing sweep for more than NW_LDS worlds: one atomic per world
+// present in a wave.
+__global__ __launch_bounds__(256) void k_world_place_wide(const Ctrl* c, Bufs b0, Bufs b1, uint32_t slots,
+	uint32_t nw, uint32_t* __restrict__ wg, uint32_t* __restrict__ perm)
+{
+	const uint32_t n_surv = c->n_surv, n_in = n_surv + c->n_se;
+	const Bufs& b = c->buf_in ? b1 : b0;
+	const uint32_t stride = gridDim.x * blockDim.x;
+	const uint32_t lane = __lane_id();
+	const uint32_t wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
+	for (uint32_t w0 = wave0; w0 < n_in; w0 += stride) {
+		const uint32_t i = w0 + lane;
+		const bool valid = i < n_in;
+		uint32_t world = 0;
+		if (valid)
+			world = min(fld_u(b, F_FLAGS)[in_slot(i, n_surv, slots)] >> FL_WORLD_SHIFT, nw - 1u);
+		uint64_t left = __ballot(valid);
+		while (left) {
+			const int first = __builtin_ctzll(left);
+			const uint32_t w = __builtin_amdgcn_readlane(world, first);
+			const uint64_t m = __ballot(valid && world == w);
+			left &= ~m;
+			uint32_t base = 0;
+			if ((int)lane == first)
+				base = wg[nw + w] + atomicAdd(&wg[2 * nw + w], (uint32_t)__popcll(m));
+			base = __builtin_amdgcn_readlane(base, first);
+			if ((m >> lane) & 1ull)
+				perm[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+		}
+	}
+}
+
 // Move planned primaries from the queue into the particle buffer
 // (gpu_driver.inl:367-389 / gpu_particle_manager.inl:342-353: vacuum
 // material, no last triangle).  Without a tag array a primary's tag is its
@@ -404,9 +580,16 @@ __device__ __forceinline__ void deposit(const StepArgs& a, Ctrl* C, bool rec, co
 	}
 }
 
-template <bool SC, bool MULTI>
+// WUNI: several worlds, one per wave.  A wave steps the particles of its
+// first lane's world (its geometry and materials are wave-uniform, so the
+// plain kernel's shared walk, walk skip and register budget apply); lanes
+// holding another world's particle wait, unchanged, for a later launch.
+// Trajectories depend only on each particle's own stream, so which launch
+// steps a particle changes no result.
+template <bool SC, bool MULTI, bool WUNI = false>
 __device__ __forceinline__ void step_body(const StepArgs& a)
 {
+	static_assert(!(WUNI && (SC || MULTI)), "the world-per-wave variant is the plain one");
 	// energy deposits are recorded by the general variants only
 	constexpr bool DEP_OK = SC || MULTI;
 	Ctrl* C = a.ctrl;
@@ -417,7 +600,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	const uint32_t B = a.slots, det_cap = a.det_cap, K = a.K;
 	const int np = a.map.np;
 	World Wl = a.W;
-	const World& W = MULTI ? Wl : a.W;
+	const World& W = (MULTI || WUNI) ? Wl : a.W;
 	__shared__ int s_path[PATH_LEVELS * 256];
 	__shared__ int s_owner[4 * 256];   // per wave: owner table, pair queue, safeties
 	// In the plain variant a particle's stream id waits here during its steps
@@ -450,13 +633,16 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	// shares its triangle tests across the wave (propagate_wave), so every
 	// lane joins it each step, also lanes whose particle has stopped.
 	const uint32_t wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
-	for (uint32_t w0 = wave0; w0 < n_in; w0 += stride) {
+	// world-per-wave: index i names particle perm[i] (~0u: padding)
+	const uint32_t n_loop = WUNI ? *a.perm_n : n_in;
+	for (uint32_t w0 = wave0; w0 < n_loop; w0 += stride) {
 		const uint32_t i = w0 + __lane_id();
-		const bool valid = i < n_in;
+		const uint32_t li = WUNI ? ((i < n_loop) ? a.perm[i] : ~0u) : i;
+		const bool valid = WUNI ? li != ~0u : i < n_in;
 		State s = {};
 		Score sc;
 		if (valid) {
-			const uint32_t slot = (i < n_surv) ? i : (B - 1u - (i - n_surv));
+			const uint32_t slot = (li < n_surv) ? li : (B - 1u - (li - n_surv));
 			s = load_state(in, slot);
 			if (SC) load_score(in, slot, np, sc);
 			if (MULTI) {
@@ -466,13 +652,35 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 				Wl.nmats = wd.nmats;
 			}
 		}
+		bool mine = valid;
+		if (WUNI) {
+			// lane 0 holds a particle (each world's range starts at a multiple
+			// of 64 and has no padding before its last particle): its world is
+			// the wave's, and no lane holds another world's particle
+			const uint32_t ww = __builtin_amdgcn_readfirstlane(s.flags >> FL_WORLD_SHIFT);
+			// through the constant address space: scalar loads, so the
+			// world's geometry and materials sit in SGPRs as the plain
+			// kernel's arguments do (not in VGPRs)
+			static_assert(sizeof(WorldDesc) % 4 == 0, "world descriptor in words");
+			using CWord = const __attribute__((address_space(4))) uint32_t;
+			CWord* src = (CWord*)(a.worlds + ww);
+			WorldDesc wd;
+			uint32_t* dst = reinterpret_cast<uint32_t*>(&wd);
+#pragma unroll
+			for (int k = 0; k < (int)(sizeof(WorldDesc) / 4); ++k)
+				dst[k] = src[k];
+			Wl.g = wd.g;
+			Wl.mats = wd.mats;
+			Wl.nmats = wd.nmats;
+			mine = valid && (s.flags >> FL_WORLD_SHIFT) == ww;
+		}
 		if (SID_LDS) {
 			s_sid[threadIdx.x] = s.sid;
 			aux[0] = s.tag;
 			aux[256] = s.step;
 			aux[512] = s.flags;
 		}
-		bool run = valid, det = false, to_inel = false, dead = false;
+		bool run = mine, det = false, to_inel = false, dead = false;
 		float rem = SKIP ? s.safe : -1.0f;
 		for (uint32_t k = 0; k < K; ++k) {
 			if (__ballot(run) == 0ull)
@@ -608,16 +816,22 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 #ifndef NBL_PLAIN_WAVES
 #define NBL_PLAIN_WAVES 5
 #endif
-template <bool SC, bool MULTI>
+template <bool SC, bool MULTI, bool WUNI = false>
 __global__ __launch_bounds__(256) void k_step(StepArgs a)
 {
-	step_body<SC, MULTI>(a);
+	step_body<SC, MULTI, WUNI>(a);
 }
 template <>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NBL_PLAIN_WAVES, 8)))
-void k_step<false, false>(StepArgs a)
+void k_step<false, false, false>(StepArgs a)
 {
-	step_body<false, false>(a);
+	step_body<false, false, false>(a);
+}
+template <>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NBL_PLAIN_WAVES, 8)))
+void k_step<false, false, true>(StepArgs a)
+{
+	step_body<false, false, true>(a);
 }
 
 // The inelastic events queued by k_step (drivers/gpu/gpu_driver.inl:505-540):
@@ -924,6 +1138,8 @@ struct nbl_driver {
 	std::vector<Material*> world_mats; // device material arrays of worlds 1..
 	std::vector<std::vector<Material>> h_world_mats;   // their host copies
 	WorldDesc* d_worlds;
+	uint32_t* wgrp;         // world grouping: counts, starts, fill levels (3 per world)
+	uint32_t* perm;         // world grouping: the step launch's indices by world
 	uint32_t next_tag;
 	ScoreMap map;           // map.np == 0: no scores
 	double* sc_mem[4];      // score arrays of buf[0], buf[1], spill, inel
@@ -1349,6 +1565,8 @@ int nbl_driver_destroy(nbl_driver* d)
 	for (Material* p : d->world_mats)
 		hipFree(p);
 	hipFree(d->d_worlds);
+	hipFree(d->wgrp);
+	hipFree(d->perm);
 	hipFree(d->dep);
 	for (double* p : d->sc_mem)
 		hipFree(p);
@@ -1494,6 +1712,20 @@ int nbl_driver_add_world(nbl_driver* d, const nbl_geometry* geom, const nbl_mate
 	HIP_OK(hipMemcpy(nw, d->h_worlds.data(), sizeof(WorldDesc) * d->h_worlds.size(), hipMemcpyHostToDevice));
 	hipFree(d->d_worlds);
 	d->d_worlds = nw;
+	// the world grouping's counters (cleared here, then by k_world_scan)
+	// and its index array
+	uint32_t* ng = nullptr;
+	if ((rc = dev_alloc(&ng, 3 * d->h_worlds.size() + 1)))
+		return rc;
+	HIP_OK(hipMemset(ng, 0, sizeof(uint32_t) * (3 * d->h_worlds.size() + 1)));
+	hipFree(d->wgrp);
+	d->wgrp = ng;
+	// every world's range padded to a multiple of 64
+	uint32_t* np = nullptr;
+	if ((rc = dev_alloc(&np, (size_t)d->slots + 64 * d->h_worlds.size())))
+		return rc;
+	hipFree(d->perm);
+	d->perm = np;
 	d->epoch++;
 	*world = (int32_t)d->h_worlds.size() - 1;
 	return NBL_OK;
@@ -1583,7 +1815,23 @@ void launch_step_pair(nbl_driver* d, const StepArgs& a, uint32_t g)
 {
 	const bool multi = d->h_worlds.size() > 1 || d->dep;
 	const dim3 grid(g), block(256);
-	if (d->map.np && multi) {
+	if (multi && !d->dep && !d->map.np) {
+		// several worlds, nothing but records: the launch's particles
+		// grouped by world, one world per wave
+		const uint32_t nw = (uint32_t)d->h_worlds.size();
+		const dim3 ggrid(std::min<uint32_t>(g, 4u * (uint32_t)d->dev->n_cu));
+		hipLaunchKernelGGL((k_world_group<false>), ggrid, block, 0, d->stream, d->ctrl, d->buf[0], d->buf[1],
+			d->slots, nw, d->wgrp, d->perm);
+		hipLaunchKernelGGL(k_world_scan, dim3(1), dim3(256), 0, d->stream, nw, d->wgrp, d->perm);
+		if (nw <= NW_LDS)
+			hipLaunchKernelGGL((k_world_group<true>), ggrid, block, 0, d->stream, d->ctrl, d->buf[0], d->buf[1],
+				d->slots, nw, d->wgrp, d->perm);
+		else
+			hipLaunchKernelGGL(k_world_place_wide, grid, block, 0, d->stream, d->ctrl, d->buf[0], d->buf[1],
+				d->slots, nw, d->wgrp, d->perm);
+		hipLaunchKernelGGL((k_step<false, false, true>), grid, block, 0, d->stream, a);
+		hipLaunchKernelGGL((k_inel<false, true>), grid, block, 0, d->stream, a);
+	} else if (d->map.np && multi) {
 		hipLaunchKernelGGL((k_step<true, true>), grid, block, 0, d->stream, a);
 		hipLaunchKernelGGL((k_inel<true, true>), grid, block, 0, d->stream, a);
 	} else if (d->map.np) {
@@ -1611,6 +1859,8 @@ StepArgs step_args(const nbl_driver* d)
 	a.inel = d->inel;
 	a.ctrl = d->ctrl;
 	a.worlds = d->d_worlds;
+	a.perm = d->perm;
+	a.perm_n = d->wgrp ? d->wgrp + 3 * d->h_worlds.size() : nullptr;
 	a.det = d->det;
 	a.det_sc = d->det_sc;
 	a.dep = d->dep;

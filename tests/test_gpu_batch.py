@@ -27,8 +27,12 @@ def _keyed(det):
     return np.sort(det, order="sid")
 
 
-@pytest.mark.parametrize("capacity", [1 << 16, 256])
-def test_mixed_worlds_equal_per_world_oracle(dev, capacity):
+@pytest.mark.parametrize("capacity,graph", [(1 << 16, False), (256, False), (256, True)])
+def test_mixed_worlds_equal_per_world_oracle(dev, capacity, graph):
+    """Three worlds in one driver (the world-per-wave step kernel: each
+    launch's particles grouped by world, every world's range padded to whole
+    waves), eager and replayed from captured graphs; capacity 256 keeps
+    every world's few particles in flight together."""
     si, pmma = material("si"), material("pmma")
     g_slab = A.Geometry(dev, slab_tris())
     g_grat = A.Geometry(dev, grating_tris())
@@ -45,6 +49,7 @@ def test_mixed_worlds_equal_per_world_oracle(dev, capacity):
     d.push(p1, np.arange(len(p1), dtype=np.uint32) + 10000, sid_base=1 << 32, world=w1)
     d.push(p0, np.arange(len(p0), dtype=np.uint32), sid_base=0, world=0)
     d.push(p2, np.arange(len(p2), dtype=np.uint32) + 20000, sid_base=2 << 32, world=w2)
+    d.set_graph(graph)
     got = d.run(200000)
     o_slab = O.OracleWorld(slab_tris(), [si])
     o_grat = O.OracleWorld(grating_tris(), [si, pmma])
