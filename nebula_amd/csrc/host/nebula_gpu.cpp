@@ -312,7 +312,7 @@ int run(int argc, char** argv)
 	p.add("seed", "Random seed", 0x78c7e39b);
 	p.add("sort-primaries", "Sort primary electrons before simulation", false);
 	p.add("inelastic", "Inelastic model: penn (full_penn) or kieft (kieft_inelastic)", "penn");
-	p.add("steps-per-launch", "Steps each electron takes per kernel launch", 8);
+	p.add("steps-per-launch", "Steps each electron takes per kernel launch", 6);
 	p.add("device", "GPU ordinal", 0);
 #ifdef NBL_EDEP
 	p.add("detect-filename", "Filename for detected electrons", "detected.bin");
