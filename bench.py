@@ -431,6 +431,8 @@ def main():
     ap.add_argument("--steps-per-launch", type=int, default=6,
                     help="steps per particle per launch (6 measured fastest on the scan: 19.58 M/s against "
                          "19.41 at 5, 19.52 at 7, 18.84 at 4)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0,
+                    help="step-kernel blocks of 256 per compute unit (0: the library's default, 64)")
     ap.add_argument("--pixels", type=int, default=256)
     ap.add_argument("--per-pixel", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -490,6 +492,8 @@ def main():
     geom = A.Geometry(dev, mesh)
     gm = [A.Material(dev, A.HostMaterial(m)) for m in mats]
     drv = A.Driver(dev, args.capacity, geom, gm, seed=1, steps_per_launch=args.steps_per_launch)
+    if args.blocks_per_cu:
+        drv.set_launch(args.steps_per_launch, args.blocks_per_cu, args.capacity)
     n = len(pri)
     cap_det = 3 * n
 

@@ -562,7 +562,7 @@ class Driver:
         return out[: n.value], sc[: n.value]
 
     def run_tally(self, tag_bin: np.ndarray, n_bins: int, frame: int = 32, se_cutoff: float = 50.0,
-                  progress_s: float = 0.0, adapt_k: bool = False):
+                  progress_s: float = 0.0, adapt_k: bool = False, k_choices=(4, 8, 16, 32)):
         """Track everything pushed to the end without keeping the records:
         counts[bin, 0] = detected electrons below `se_cutoff` eV (SE) and
         counts[bin, 1] = the others (BSE), binned by tag_bin[tag].  The
@@ -572,7 +572,7 @@ class Driver:
         adapt_k: follow the best steps per launch as the run goes on.  The
         best value drifts (short launches win while the buffer is dense,
         long ones once few particles are left), so every fourth frame
-        probes a neighbouring value in {4, 8, 16, 32}, and the run moves to
+        probes a neighbouring value in `k_choices`, and the run moves to
         it when its particle-steps per second beat the mean of the frames
         on either side by more than 3 %.  Trajectories do not depend on the
         launch configuration, so neither do the counts."""
@@ -583,9 +583,9 @@ class Driver:
         buf = np.zeros(max(1, self.capacity), dtype=DET_DTYPE)
         n = C.c_uint32()
         last_steps, stalled = -1, 0
-        ks = (4, 8, 16, 32)
+        ks = tuple(k_choices)
         k0 = self.launch["steps_per_launch"]
-        ki = ks.index(k0) if k0 in ks else 1
+        ki = ks.index(k0) if k0 in ks else min(1, len(ks) - 1)
         frame_no, direction, probe, pending, before = 0, +1, None, None, None
         steps_prev = self.steps()
         self.k_frames = {}  # frames run at each steps-per-launch value

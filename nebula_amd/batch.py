@@ -33,7 +33,7 @@ def geometries(dev: A.Device, scans):
 
 def run_batched(dev: A.Device, scans, materials, capacity: int = 1 << 21, seed: int = 1,
                 autotune: bool = False, graph: bool = True, iters_per_trial: int = 4, geoms=None,
-                progress_s: float = 0.0, adapt_k: bool = True):
+                progress_s: float = 0.0, adapt_k: bool = True, k_choices=(4, 8, 16, 32)):
     """All scans in one driver.  Returns (counts [n_scans, 2] of SE / BSE
     detections, seconds of tracking, tuning result or None).  Geometry and
     material upload and the primaries' transfer to HBM precede the clock.
@@ -52,7 +52,7 @@ def run_batched(dev: A.Device, scans, materials, capacity: int = 1 << 21, seed: 
     dev.synchronize()
     t0 = time.perf_counter()
     tune = d.autotune(iters_per_trial) if autotune else None
-    counts = d.run_tally(scan_of_tag, len(scans), progress_s=progress_s, adapt_k=adapt_k)
+    counts = d.run_tally(scan_of_tag, len(scans), progress_s=progress_s, adapt_k=adapt_k, k_choices=k_choices)
     if adapt_k:
         tune = dict(tune or {}, frames_per_steps_per_launch=dict(sorted(d.k_frames.items())))
     dev.synchronize()
