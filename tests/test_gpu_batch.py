@@ -175,3 +175,26 @@ def test_material_upload_refuses_incomplete_tables(dev):
         t.ndim, t.dims[0], t.dims[1] = saved[0], saved[2], saved[3]
         t.data = C.cast(C.c_void_p(saved[1]), C.POINTER(C.c_float))
         A.Material(dev, hm)   # the restored material uploads
+
+
+def test_many_worlds_equal_one_world(dev):
+    """1100 worlds, more than the world grouping's LDS histogram holds (so
+    the counting and placing take the one-atomic-per-wave path), all with
+    the same geometry and materials: the records equal those of one world
+    holding the same primaries, stream id for stream id."""
+    g = A.Geometry(dev, slab_tris())
+    m = [A.Material(dev, A.HostMaterial(material("si")))]
+    pri = W.pencil_beam(2200, 1000.0)
+    one = A.Driver(dev, 4096, g, m, seed=21, steps_per_launch=6)
+    one.push(pri)
+    ref = _keyed(one.run(200000))
+    one.close()
+    d = A.Driver(dev, 4096, g, m, seed=21, steps_per_launch=6)
+    worlds = [0] + [d.add_world(g, m) for _ in range(1099)]
+    for w in worlds:
+        d.push(pri[2 * w: 2 * w + 2], sid_base=2 * w, world=w)
+    got = _keyed(d.run(200000))
+    d.close()
+    print(f"{len(worlds)} worlds: {len(got)} records")
+    assert len(ref) > 500
+    assert got.tobytes() == ref.tobytes()
