@@ -200,9 +200,16 @@ void pack_records(const nbl_detected* det, uint32_t n, const std::vector<nbl_rec
 	}
 	std::vector<std::thread> th;
 	const uint32_t chunk = (n + nt - 1) / nt;
-	for (uint32_t t = 1; t < nt; ++t)
-		th.emplace_back(pack, std::min(n, t * chunk), std::min(n, (t + 1) * chunk));
+	// the chunks without a thread (all but the first, if none could be
+	// started) are packed here; started threads are always joined
+	uint32_t started = 1;
+	try {
+		for (; started < nt; ++started)
+			th.emplace_back(pack, std::min(n, started * chunk), std::min(n, (started + 1) * chunk));
+	} catch (...) {
+	}
 	pack(0, std::min(n, chunk));
+	pack(std::min(n, started * chunk), n);
 	for (auto& x : th)
 		x.join();
 }

@@ -32,7 +32,7 @@ def _expected(mesh, pri, prescan, seed, sort=False):
     rec = A.sort_primaries(pri) if sort else pri
     rec = A.prescan_shuffle(rec, prescan)
     ow = O.OracleWorld(A.mesh_to_triangles(mesh), [material("si"), material("pmma")])
-    det, _ = ow.run_philox(rec, _mt_first(seed), 200000, threads=8)
+    det, _ = ow.run_philox(rec, _mt_first(seed), 2 * len(rec) + 100000, threads=8)
     out = np.zeros(len(det), W.PRI_DTYPE)
     out["pos"], out["dir"], out["E"] = det["pos"], det["dir"], det["E"]
     out["px"] = rec["px"][det["tag"]]
@@ -45,18 +45,21 @@ def _as_sorted_bytes(rec):
     return np.sort(v)
 
 
-@pytest.mark.parametrize("capacity,prescan,sort", [(65536, 100, 0), (512, 64, 1)])
-def test_cli_records_equal_oracle(tmp_path, capacity, prescan, sort):
+@pytest.mark.parametrize("capacity,prescan,sort,side,per_px", [(65536, 100, 0, 6, 10), (512, 64, 1, 6, 10),
+                                                                (1 << 20, 1000, 0, 64, 100)])
+def test_cli_records_equal_oracle(tmp_path, capacity, prescan, sort, side, per_px):
     """Small capacity forces the frame loop through many refills and
-    secondary-slot back-pressure; the output must not change."""
+    secondary-slot back-pressure; the output must not change.  The large
+    case (409 600 primaries) puts more than 65 536 records in a frame, so
+    the binary converts them on several threads."""
     mesh = W.grating_geometry(dy=3.2)
-    pri = W.raster_scan(6, 6, 10, 300.0, -48.0, 48.0, -48.0, 48.0, 37.0, seed=5)
+    pri = W.raster_scan(side, side, per_px, 300.0, -48.0, 48.0, -48.0, 48.0, 37.0, seed=5)
     W.write_tri(str(tmp_path / "g.tri"), mesh)
     W.write_pri(str(tmp_path / "s.pri"), pri)
     A.write_mat_file(str(tmp_path / "si.mat"), material("si"))
     A.write_mat_file(str(tmp_path / "pmma.mat"), material("pmma"))
     seed = 7
-    print(f"nebula_gpu capacity={capacity} prescan={prescan} sort={sort}", flush=True)
+    print(f"nebula_gpu capacity={capacity} prescan={prescan} sort={sort} primaries={len(pri)}", flush=True)
     r = subprocess.run([CLI, "--capacity", str(capacity), "--prescan-size", str(prescan),
                         "--seed", str(seed), "--sort-primaries", str(sort),
                         str(tmp_path / "g.tri"), str(tmp_path / "s.pri"),
