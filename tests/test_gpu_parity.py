@@ -226,7 +226,16 @@ FULL_SIZE = {
     "si_slab_1e5_1keV": lambda: (slab_tris(), W.pencil_beam(100000, 1000.0), ["si"], 1),
     # a 40 x 40-pixel crop (100 primaries per pixel, 300 eV) of the bench scan
     "grating_crop_40x40x100_300eV": lambda: _grating_crop(),
+    # the whole bench scan, 6.55 M primaries: about 80 s of oracle time on
+    # 16 cores, so it runs only with NBL_FULL_SCAN=1
+    "bench_scan_256x256x100_300eV": lambda: _grating_scan(),
 }
+
+
+def _grating_scan():
+    mesh = W.grating_geometry(dy=3.2, seed=1)
+    pri = W.raster_scan(256, 256, 100, 300.0, -128.0, 128.0, -128.0, 128.0, 37.0, spot_sigma=1.0, seed=1)
+    return A.mesh_to_triangles(mesh), pri, ["si", "pmma"], 1
 
 
 def _grating_crop():
@@ -241,17 +250,27 @@ def test_full_size_cascades_equal_oracle(dev, case):
     """Whole configurations, not samples: every detected electron of the
     GPU run equals the oracle's, bit for bit, and so does the step count."""
     import os
+    if case.startswith("bench_scan") and os.environ.get("NBL_FULL_SCAN") != "1":
+        pytest.skip("the whole bench scan takes about 80 s of oracle time: set NBL_FULL_SCAN=1")
     tris, pri, mnames, seed = FULL_SIZE[case]()
     raws = [material(m) for m in mnames]
     g = A.Geometry(dev, tris)
-    d = A.Driver(dev, 1 << 20, g, [A.Material(dev, A.HostMaterial(r)) for r in raws], seed=seed,
+    d = A.Driver(dev, 1 << 22, g, [A.Material(dev, A.HostMaterial(r)) for r in raws], seed=seed,
                  steps_per_launch=6)
     d.push(pri)
     det_g = d.run(50 * len(pri))
     steps_g = d.steps()
     d.close()
-    det_o, steps_o = O.OracleWorld(tris, raws).run_philox(pri, seed, 50 * len(pri),
-                                                          threads=min(16, os.cpu_count() or 1))
+    ow = O.OracleWorld(tris, raws)
+    parts, steps_o = [], 0
+    for k in range(0, len(pri), 409600):
+        part = pri[k:k + 409600]
+        det, st = ow.run_philox(part, seed, 50 * len(part), threads=min(16, os.cpu_count() or 1),
+                                sid_base=k, tag_base=k)
+        parts.append(det)
+        steps_o += st
+        print(f"oracle: {k + len(part)} of {len(pri)} primaries", flush=True)
+    det_o = np.concatenate(parts)
     n_common, n_same, n_only = _compare_detected(det_g, det_o)
     print(f"{case}: {len(pri)} primaries, {steps_g} steps, {len(det_g)} records "
           f"(oracle {len(det_o)}, common {n_common}, same {n_same}, only one side {n_only})")
