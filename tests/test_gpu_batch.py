@@ -198,3 +198,46 @@ def test_many_worlds_equal_one_world(dev):
     print(f"{len(worlds)} worlds: {len(got)} records")
     assert len(ref) > 500
     assert got.tobytes() == ref.tobytes()
+
+
+def test_whole_batch_equals_oracle(dev):
+    """The 64-scan batch at scale 0.1 (13.7 M primaries, 1e3-2e5 triangles,
+    200 eV-20 keV) through one driver with 64 worlds, against the oracle
+    scan by scan: every record and the step count equal.  About 15 minutes
+    of oracle time on 16 cores, so it runs only with NBL_FULL_BATCH=1."""
+    import os
+    if os.environ.get("NBL_FULL_BATCH") != "1":
+        pytest.skip("the whole batch takes about 15 minutes of oracle time: set NBL_FULL_BATCH=1")
+    from nebula_amd import batch as B
+    from tests.test_gpu_parity import _compare_detected
+    scans = W.batch_scans(64, seed=1, per_pixel_scale=0.1)
+    raws = [material("si"), material("pmma")]
+    mats = [A.Material(dev, A.HostMaterial(m)) for m in raws]
+    geoms = B.geometries(dev, scans)
+    offs, _ = B._tags(scans)
+    n = int(offs[-1])
+    d = A.Driver(dev, 1 << 21, geoms[0], mats, seed=1, steps_per_launch=8)
+    worlds = [0] + [d.add_world(g, mats) for g in geoms[1:]]
+    for i, s in enumerate(scans):
+        m = len(s["primaries"])
+        d.push(s["primaries"], np.arange(offs[i], offs[i] + m, dtype=np.uint32), sid_base=i << 40,
+               world=worlds[i])
+    det_g = d.run(3 * n)
+    steps_g = d.steps()
+    d.close()
+    parts, steps_o = [], 0
+    for i, s in enumerate(scans):
+        ow = O.OracleWorld(A.mesh_to_triangles(s["mesh"]), raws)
+        m = len(s["primaries"])
+        for k in range(0, m, 50000):
+            part = s["primaries"][k:k + 50000]
+            det, st = ow.run_philox(part, 1, 4 * len(part) + 10000, threads=min(16, os.cpu_count() or 1),
+                                    sid_base=(i << 40) + k, tag_base=int(offs[i]) + k)
+            parts.append(det)
+            steps_o += st
+        print(f"oracle: scan {i + 1} of 64", flush=True)
+    det_o = np.concatenate(parts)
+    n_common, n_same, n_only = _compare_detected(det_g, det_o)
+    print(f"{n} primaries, {steps_g} steps, {len(det_g)} records (oracle {len(det_o)}, same {n_same})")
+    assert steps_g == steps_o
+    assert n_only == 0 and n_common == n_same == len(det_o) == len(det_g)
