@@ -244,6 +244,37 @@ def bench_batched(dev, scale=1.0):
             "speedup": tp / tb, "identical_yields": bool(np.array_equal(cb, cp)), "tune": tune}
 
 
+def bench_batch_classes(dev, scale=1.0, bounds=(5e5, 2e6)):
+    """The 64-scan workload split by scan size (primaries at this scale:
+    below bounds[0], up to bounds[1], above): each class batched in one
+    driver and run scan by scan, as in bench_batched."""
+    from nebula_amd import batch as B
+    scans = W.batch_scans(64, seed=1, per_pixel_scale=scale)
+    mats = [A.Material(dev, A.HostMaterial(m)) for m in (W.make_material(W.SI, "penn"),
+                                                          W.make_material(W.PMMA, "penn"))]
+    t0 = time.perf_counter()
+    geoms = B.geometries(dev, scans)
+    print(f"octrees built and uploaded in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    sizes = np.array([len(s["primaries"]) for s in scans])
+    edges = (0.0,) + tuple(float(b) * scale for b in bounds) + (float("inf"),)
+    out = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        idx = [i for i in range(len(scans)) if lo <= sizes[i] < hi]
+        if not idx:
+            continue
+        sub, sg = [scans[i] for i in idx], [geoms[i] for i in idx]
+        n = int(sizes[idx].sum())
+        cb, tb, _ = B.run_batched(dev, sub, mats, capacity=1 << 21, seed=1, geoms=sg)
+        cp, tp = B.run_plain(dev, sub, mats, capacity=1 << 21, seed=1, geoms=sg)
+        r = {"primaries_per_scan": [int(sizes[idx].min()), int(sizes[idx].max())], "scans": len(idx),
+             "primaries": n, "batched_s": tb, "plain_s": tp, "speedup": tp / tb,
+             "identical_yields": bool(np.array_equal(cb, cp))}
+        print(f"class {r['primaries_per_scan']}: {len(idx)} scans, batched {tb:.2f} s, plain {tp:.2f} s",
+              file=sys.stderr, flush=True)
+        out.append(r)
+    return out
+
+
 def bench_large(dev, scale=1.0, chunk=5_000_000):
     """The large configuration: a 20x20 array of contact holes (40 nm
     diameter, 80 nm deep through a PMMA layer and a SiO2-like layer on Si,
@@ -450,6 +481,9 @@ def main():
     ap.add_argument("--grad-reference", type=int, default=0,
                     help="run only the gradient check against a reference gradient from this many "
                          "times the primaries (fresh replicas of the 1e6-primary scan); print its JSON line")
+    ap.add_argument("--batch-classes", action="store_true",
+                    help="run only the batched comparison per scan-size class (at --batch-scale) "
+                         "and print its JSON line")
     ap.add_argument("--cli", action="store_true",
                     help="run only the command-line drop-in (nebula_gpu) on the scan and print its JSON line")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "step_pair_traffic.json"),
@@ -465,6 +499,13 @@ def main():
                           "value": r["primaries_per_s"], "unit": "primaries/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "f32", "data": "synthetic", "large": r}),
               flush=True)
+        return
+    if args.batch_classes:
+        r = bench_batch_classes(A.Device(0), args.batch_scale)
+        print(json.dumps({"metric": "batched / plain speed-up per scan-size class",
+                          "value": min(c["speedup"] for c in r), "unit": "x (smallest over the classes)",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+                          "batch_scale": args.batch_scale, "classes": r}), flush=True)
         return
     if args.cli:
         r = bench_cli(args.capacity, args.steps_per_launch, args.pixels, args.per_pixel)

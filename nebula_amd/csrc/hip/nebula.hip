@@ -119,7 +119,7 @@ struct StepArgs {
 	// world-per-wave variant: the coming launch's particles grouped by
 	// world (k_world_place); k_step's wave takes perm[i] for index i
 	const uint32_t* perm;
-	const uint32_t* perm_n;   // its padded length
+	const uint32_t* perm_n;   // its padded length; perm_n[1]: one world, perm unused
 };
 
 __device__ __forceinline__ State load_state(const Bufs& b, uint32_t i)
@@ -299,10 +299,14 @@ __device__ __forceinline__ uint32_t in_slot(uint32_t i, uint32_t n_surv, uint32_
 // k_world_group<true> writes each index into its world's range, a wave's
 // lanes of one world together, so that the step kernel's waves read runs
 // of neighbouring slots.  wg holds counts [0, nw), starts [nw, 2 nw), fill
-// levels [2 nw, 3 nw) and the padded length of the index array at 3 nw.
+// levels [2 nw, 3 nw), the padded length of the index array at 3 nw, and
+// at 3 nw + 1 a flag set when at most one world has particles: the
+// launch's indices are then grouped as they stand, the placing pass
+// returns at once and the step kernel reads index i as itself.
 // Exclusive prefix of the counts into the starts (each padded to a
 // multiple of 64, the padding marked ~0u), the padded length at wg[3 nw],
-// counters and fill levels cleared for the next launch.
+// the one-world flag at wg[3 nw + 1], counters and fill levels cleared for
+// the next launch.
 __device__ void world_scan(uint32_t nw, uint32_t* __restrict__ wg, uint32_t* __restrict__ perm)
 {
 	__shared__ uint32_t s_v[256];
@@ -310,10 +314,12 @@ __device__ void world_scan(uint32_t nw, uint32_t* __restrict__ wg, uint32_t* __r
 	if (threadIdx.x == 0)
 		s_carry = 0;
 	__syncthreads();
+	uint32_t present = 0;   // worlds with particles
 	for (uint32_t base = 0; base < nw; base += blockDim.x) {
 		const uint32_t i = base + threadIdx.x;
 		const uint32_t n = (i < nw) ? wg[i] : 0u;
 		const uint32_t v = (n + 63u) & ~63u;
+		present += (uint32_t)__syncthreads_count(n != 0u);
 		s_v[threadIdx.x] = v;
 		__syncthreads();
 		for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
@@ -335,8 +341,10 @@ __device__ void world_scan(uint32_t nw, uint32_t* __restrict__ wg, uint32_t* __r
 			s_carry += s_v[threadIdx.x];
 		__syncthreads();
 	}
-	if (threadIdx.x == 0)
+	if (threadIdx.x == 0) {
 		wg[3 * nw] = s_carry;
+		wg[3 * nw + 1] = (present <= 1u) ? 1u : 0u;
+	}
 }
 
 // Each block takes a contiguous chunk of the indices (whole waves) and
@@ -353,6 +361,8 @@ __global__ __launch_bounds__(256) void k_world_group(const Ctrl* c, Bufs b0, Buf
 	uint32_t nw, uint32_t* __restrict__ wg, uint32_t* __restrict__ perm)
 {
 	__shared__ uint32_t s_h[NW_LDS];
+	if (PLACE && wg[3 * nw + 1])
+		return;   // one world: nothing to place
 	const uint32_t n_surv = c->n_surv, n_in = n_surv + c->n_se;
 	const Bufs& b = c->buf_in ? b1 : b0;
 	const uint32_t lane = __lane_id();
@@ -429,6 +439,8 @@ __global__ __launch_bounds__(256) void k_world_scan(uint32_t nw, uint32_t* __res
 __global__ __launch_bounds__(256) void k_world_place_wide(const Ctrl* c, Bufs b0, Bufs b1, uint32_t slots,
 	uint32_t nw, uint32_t* __restrict__ wg, uint32_t* __restrict__ perm)
 {
+	if (wg[3 * nw + 1])
+		return;   // one world: nothing to place
 	const uint32_t n_surv = c->n_surv, n_in = n_surv + c->n_se;
 	const Bufs& b = c->buf_in ? b1 : b0;
 	const uint32_t stride = gridDim.x * blockDim.x;
@@ -633,12 +645,14 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 	// shares its triangle tests across the wave (propagate_wave), so every
 	// lane joins it each step, also lanes whose particle has stopped.
 	const uint32_t wave0 = __builtin_amdgcn_readfirstlane(blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
-	// world-per-wave: index i names particle perm[i] (~0u: padding)
-	const uint32_t n_loop = WUNI ? *a.perm_n : n_in;
+	// world-per-wave: index i names particle perm[i] (~0u: padding), or
+	// particle i when the launch holds one world's particles only
+	const bool ident = WUNI && a.perm_n[1] != 0u;
+	const uint32_t n_loop = (WUNI && !ident) ? *a.perm_n : n_in;
 	for (uint32_t w0 = wave0; w0 < n_loop; w0 += stride) {
 		const uint32_t i = w0 + __lane_id();
-		const uint32_t li = WUNI ? ((i < n_loop) ? a.perm[i] : ~0u) : i;
-		const bool valid = WUNI ? li != ~0u : i < n_in;
+		const uint32_t li = (WUNI && !ident) ? ((i < n_loop) ? a.perm[i] : ~0u) : i;
+		const bool valid = (WUNI && !ident) ? li != ~0u : i < n_in;
 		State s = {};
 		Score sc;
 		if (valid) {
@@ -655,8 +669,9 @@ __device__ __forceinline__ void step_body(const StepArgs& a)
 		bool mine = valid;
 		if (WUNI) {
 			// lane 0 holds a particle (each world's range starts at a multiple
-			// of 64 and has no padding before its last particle): its world is
-			// the wave's, and no lane holds another world's particle
+			// of 64 and has no padding before its last particle; with one
+			// world, w0 < n_in): its world is the wave's, and no lane holds
+			// another world's particle
 			const uint32_t ww = __builtin_amdgcn_readfirstlane(s.flags >> FL_WORLD_SHIFT);
 			// through the constant address space: scalar loads, so the
 			// world's geometry and materials sit in SGPRs as the plain
@@ -1715,9 +1730,9 @@ int nbl_driver_add_world(nbl_driver* d, const nbl_geometry* geom, const nbl_mate
 	// the world grouping's counters (cleared here, then by k_world_scan)
 	// and its index array
 	uint32_t* ng = nullptr;
-	if ((rc = dev_alloc(&ng, 3 * d->h_worlds.size() + 1)))
+	if ((rc = dev_alloc(&ng, 3 * d->h_worlds.size() + 2)))
 		return rc;
-	HIP_OK(hipMemset(ng, 0, sizeof(uint32_t) * (3 * d->h_worlds.size() + 1)));
+	HIP_OK(hipMemset(ng, 0, sizeof(uint32_t) * (3 * d->h_worlds.size() + 2)));
 	hipFree(d->wgrp);
 	d->wgrp = ng;
 	// every world's range padded to a multiple of 64
