@@ -63,6 +63,28 @@ def test_mixed_worlds_equal_per_world_oracle(dev, capacity, graph):
     assert _keyed(got).tobytes() == _keyed(exp).tobytes()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_one_world_in_flight_equals_oracle(dev, graph):
+    """Two worlds, every particle in world 1: each launch holds one world's
+    particles, so the grouping skips the placing pass and the step kernel
+    reads its particles in place; their world (the grating, not world 0's
+    slab) must still be the one they track in."""
+    si, pmma = material("si"), material("pmma")
+    m_si = A.Material(dev, A.HostMaterial(si))
+    m_pm = A.Material(dev, A.HostMaterial(pmma))
+    d = A.Driver(dev, 1 << 12, A.Geometry(dev, slab_tris()), [m_si], seed=17, steps_per_launch=8)
+    w1 = d.add_world(A.Geometry(dev, grating_tris()), [m_si, m_pm])
+    p = W.raster_scan(6, 6, 20, 500.0, -32.0, 32.0, -32.0, 32.0, 37.0, seed=4)
+    d.push(p, np.arange(len(p), dtype=np.uint32), sid_base=5 << 32, world=w1)
+    d.set_graph(graph)
+    got = d.run(200000)
+    exp, _ = O.OracleWorld(grating_tris(), [si, pmma]).run_philox(p, 17, 100000, threads=8,
+                                                                  sid_base=5 << 32, tag_base=0)
+    assert len(exp) > 50
+    assert len(got) == len(exp)
+    assert _keyed(got).tobytes() == _keyed(exp).tobytes()
+
+
 def test_bad_world_is_rejected(dev):
     g = A.Geometry(dev, slab_tris())
     d = A.Driver(dev, 1024, g, [A.Material(dev, A.HostMaterial(material("si")))], seed=1)
