@@ -117,7 +117,7 @@ struct StepArgs {
 	uint32_t spill_cap;
 	uint32_t K;
 	// world-per-wave variant: the coming launch's particles grouped by
-	// world (k_world_place); k_step's wave takes perm[i] for index i
+	// world (k_world_group); k_step's wave takes perm[i] for index i
 	const uint32_t* perm;
 	const uint32_t* perm_n;   // its padded length; perm_n[1]: one world, perm unused
 };
