@@ -244,10 +244,12 @@ def bench_batched(dev, scale=1.0):
             "speedup": tp / tb, "identical_yields": bool(np.array_equal(cb, cp)), "tune": tune}
 
 
-def bench_batch_classes(dev, scale=1.0, bounds=(5e5, 2e6)):
+def bench_batch_classes(dev, scale=1.0, bounds=(5e5, 2e6), only=None, graph=True):
     """The 64-scan workload split by scan size (primaries at this scale:
     below bounds[0], up to bounds[1], above): each class batched in one
-    driver and run scan by scan, as in bench_batched."""
+    driver and run scan by scan, as in bench_batched.  only: run that class
+    alone (0, 1, 2); graph=False: the batched run launches eagerly (for
+    kernel traces)."""
     from nebula_amd import batch as B
     scans = W.batch_scans(64, seed=1, per_pixel_scale=scale)
     mats = [A.Material(dev, A.HostMaterial(m)) for m in (W.make_material(W.SI, "penn"),
@@ -258,13 +260,13 @@ def bench_batch_classes(dev, scale=1.0, bounds=(5e5, 2e6)):
     sizes = np.array([len(s["primaries"]) for s in scans])
     edges = (0.0,) + tuple(float(b) * scale for b in bounds) + (float("inf"),)
     out = []
-    for lo, hi in zip(edges[:-1], edges[1:]):
+    for c, (lo, hi) in enumerate(zip(edges[:-1], edges[1:])):
         idx = [i for i in range(len(scans)) if lo <= sizes[i] < hi]
-        if not idx:
+        if not idx or (only is not None and c != only):
             continue
         sub, sg = [scans[i] for i in idx], [geoms[i] for i in idx]
         n = int(sizes[idx].sum())
-        cb, tb, _ = B.run_batched(dev, sub, mats, capacity=1 << 21, seed=1, geoms=sg)
+        cb, tb, _ = B.run_batched(dev, sub, mats, capacity=1 << 21, seed=1, geoms=sg, graph=graph)
         cp, tp = B.run_plain(dev, sub, mats, capacity=1 << 21, seed=1, geoms=sg)
         r = {"primaries_per_scan": [int(sizes[idx].min()), int(sizes[idx].max())], "scans": len(idx),
              "primaries": n, "batched_s": tb, "plain_s": tp, "speedup": tp / tb,
@@ -484,6 +486,10 @@ def main():
     ap.add_argument("--batch-classes", action="store_true",
                     help="run only the batched comparison per scan-size class (at --batch-scale) "
                          "and print its JSON line")
+    ap.add_argument("--batch-class", type=int, default=None,
+                    help="with --batch-classes: only this class (0 small, 1 medium, 2 large)")
+    ap.add_argument("--batch-eager", action="store_true",
+                    help="with --batch-classes: batched runs launch eagerly instead of replaying graphs")
     ap.add_argument("--cli", action="store_true",
                     help="run only the command-line drop-in (nebula_gpu) on the scan and print its JSON line")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "step_pair_traffic.json"),
@@ -501,7 +507,8 @@ def main():
               flush=True)
         return
     if args.batch_classes:
-        r = bench_batch_classes(A.Device(0), args.batch_scale)
+        r = bench_batch_classes(A.Device(0), args.batch_scale, only=args.batch_class,
+                                graph=not args.batch_eager)
         print(json.dumps({"metric": "batched / plain speed-up per scan-size class",
                           "value": min(c["speedup"] for c in r), "unit": "x (smallest over the classes)",
                           "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
