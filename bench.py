@@ -509,6 +509,8 @@ def main():
     if args.batch_classes:
         r = bench_batch_classes(A.Device(0), args.batch_scale, only=args.batch_class,
                                 graph=not args.batch_eager)
+        if not r:
+            sys.exit("--batch-class: no scans in that class (0, 1 or 2)")
         print(json.dumps({"metric": "batched / plain speed-up per scan-size class",
                           "value": min(c["speedup"] for c in r), "unit": "x (smallest over the classes)",
                           "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
